@@ -1,0 +1,676 @@
+// hcm_api.cpp -- host side of the C ABI declared in include/hcm.h.
+//
+// Owns device memory (registry tables, quad pools, chunk stores, job tables), the HIP stream each
+// context works on, and the HIP graph that replays the batched pipeline (counter reset + persistent
+// meshing kernel). Kernels live in hcm_kernels.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "hcm_internal.h"
+
+using namespace hcm;
+
+namespace {
+
+thread_local int g_last_hip = 0;
+
+#define HCM_HIP(call)                                                                                              \
+	do {                                                                                                           \
+		hipError_t e_ = (call);                                                                                    \
+		if (e_ != hipSuccess) {                                                                                    \
+			g_last_hip = (int)e_;                                                                                  \
+			return HCM_ERR_HIP;                                                                                    \
+		}                                                                                                          \
+	} while (0)
+
+template <typename T> hipError_t dmalloc(T **p, size_t count) {
+	return hipMalloc(reinterpret_cast<void **>(p), std::max<size_t>(count, 1) * sizeof(T));
+}
+
+// Registry flattened for the kernels (see hcm_internal.h for the word layouts).
+struct DeviceTables {
+	std::vector<uint8_t> pid_of;
+	std::vector<uint2> rows;
+	std::vector<uint32_t> codes;
+	std::vector<hcm_custom_mesh> meshes;
+	std::vector<uint8_t> tex_flags;
+	uint32_t flow_pid_base = 0, flow_tex = 0, src_id = 0, flow_id = 0;
+};
+
+int build_tables(const hcm_registry *r, DeviceTables *t) {
+	if (!r || !r->row_of || !r->rows || r->n_rows == 0 || (r->n_meshes && !r->meshes) || (r->n_tex && !r->tex_flags))
+		return HCM_ERR_INVALID;
+	if (r->n_rows + kFlowPatterns > (uint32_t)kMaxPid || r->n_meshes >= 255)
+		return HCM_ERR_INVALID;
+	auto tex_flags = [&](uint16_t tex) -> uint32_t {
+		uint32_t id = tex & 0x1fffu;
+		return id < r->n_tex ? r->tex_flags[id] : 0u;
+	};
+	// texture codes: code 0 is the empty texture value 0; codes are unique per full texture value
+	std::vector<uint16_t> code_tex{0};
+	auto code_of = [&](uint16_t tex) -> int {
+		for (size_t i = 0; i < code_tex.size(); ++i)
+			if (code_tex[i] == tex)
+				return (int)i;
+		code_tex.push_back(tex);
+		return (int)code_tex.size() - 1;
+	};
+	t->rows.clear();
+	for (uint32_t i = 0; i < r->n_rows; ++i) {
+		const hcm_block_row &row = r->rows[i];
+		int tc[6];
+		for (int f = 0; f < 6; ++f)
+			tc[f] = code_of(row.tex[f]);
+		uint32_t flags = 0;
+		if (row.transparency == 2 || row.collision != 1)
+			flags |= kRowDirectPass;
+		if (row.transparency != 0)
+			flags |= kRowIndirectPass;
+		uint32_t mesh = 0xffu;
+		if (row.mesh != HCM_NO_MESH) {
+			if (row.mesh >= r->n_meshes)
+				return HCM_ERR_INVALID;
+			mesh = row.mesh;
+			flags |= kRowHasMesh;
+			if (r->meshes[row.mesh].kind == HCM_MESH_LIQUID_FLOW)
+				flags |= kRowLiquidFlow;
+			else if (r->meshes[row.mesh].face_count > HCM_MAX_FACES)
+				return HCM_ERR_INVALID;
+		}
+		t->rows.push_back(make_uint2(uint32_t(tc[0]) | uint32_t(tc[1]) << 8 | uint32_t(tc[2]) << 16 |
+		                                 uint32_t(tc[3]) << 24,
+		                             uint32_t(tc[4]) | uint32_t(tc[5]) << 8 | flags << 16 | mesh << 24));
+	}
+	// dynamic liquid-flow textures: 16 = all faces liquid; else bits right, left, front, back full
+	t->flow_pid_base = r->n_rows;
+	const uint32_t fc = (uint32_t)code_of(r->liquid_flow_tex);
+	for (uint32_t p = 0; p < (uint32_t)kFlowPatterns; ++p) {
+		uint32_t tc[6];
+		if (p == 16)
+			for (int f = 0; f < 6; ++f)
+				tc[f] = fc;
+		else {
+			tc[0] = (p & 1) ? fc : 0, tc[1] = (p & 2) ? fc : 0, tc[2] = 0, tc[3] = fc;
+			tc[4] = (p & 4) ? fc : 0, tc[5] = (p & 8) ? fc : 0;
+		}
+		t->rows.push_back(make_uint2(tc[0] | tc[1] << 8 | tc[2] << 16 | tc[3] << 24,
+		                             tc[4] | tc[5] << 8 | (kRowDirectPass | kRowIndirectPass) << 16 | 0xffu << 24));
+	}
+	if (code_tex.size() > (size_t)kMaxCodes)
+		return HCM_ERR_INVALID;
+	t->codes.clear();
+	for (uint16_t tex : code_tex) {
+		uint32_t f = tex_flags(tex), cf = 0;
+		if (f & HCM_TEX_TRANSPARENT)
+			cf |= kCodeTransparent;
+		if (f & HCM_TEX_TRANS_PASS)
+			cf |= kCodeTransPass;
+		if (f & HCM_TEX_LIQUID)
+			cf |= kCodeLiquid;
+		if ((tex & 0x1fffu) == 0)
+			cf |= kCodeEmpty;
+		t->codes.push_back(uint32_t(tex) | cf << 16);
+	}
+	t->pid_of.resize(65536);
+	for (uint32_t d = 0; d < 65536; ++d) {
+		if (r->row_of[d] >= r->n_rows)
+			return HCM_ERR_INVALID;
+		t->pid_of[d] = (uint8_t)r->row_of[d];
+	}
+	t->meshes.assign(r->meshes, r->meshes + r->n_meshes);
+	t->tex_flags.assign(r->tex_flags, r->tex_flags + r->n_tex);
+	t->flow_tex = r->liquid_flow_tex;
+	t->src_id = r->liquid_source_id;
+	t->flow_id = r->liquid_flow_id;
+	return HCM_OK;
+}
+
+constexpr uint32_t kPlainPoolQuads = 1u << 17; // > the most quads one 16^3 chunk can produce
+constexpr uint32_t kPlainRecs = 64;
+
+} // namespace
+
+struct hcm_ctx {
+	int device = 0;
+	int n_cu = 0;
+	hipStream_t stream = nullptr;
+	std::mutex mu;
+	DevReg reg{};
+	// device registry buffers
+	uint8_t *d_pid_of = nullptr;
+	uint2 *d_rows = nullptr;
+	uint32_t *d_codes = nullptr;
+	hcm_custom_mesh *d_meshes = nullptr;
+	uint8_t *d_tex_flags = nullptr;
+	// batched pool
+	uint64_t pool_quads = 0;
+	uint2 *d_verts = nullptr;
+	uint16_t *d_idx = nullptr;
+	// plain path
+	uint16_t *d_pblocks = nullptr;
+	uint8_t *d_plight = nullptr;
+	uint32_t *d_pnbr = nullptr;
+	Counters *d_pctr = nullptr;
+	uint2 *d_pverts = nullptr;
+	uint16_t *d_pidx = nullptr;
+	hcm_mesh_rec *d_precs = nullptr;
+	hcm_job_result *d_pres = nullptr;
+	uint16_t *h_pblocks = nullptr; // pinned staging
+	uint8_t *h_plight = nullptr;
+	hcm_tuning tuning{0, 256, {0, 0}};
+};
+
+struct hcm_store {
+	hcm_ctx *ctx = nullptr;
+	uint32_t n_slots = 0;
+	uint16_t *d_blocks = nullptr;
+	uint8_t *d_light = nullptr;
+};
+
+struct hcm_batch {
+	hcm_ctx *ctx = nullptr;
+	hcm_store *store = nullptr;
+	uint32_t n_jobs = 0;
+	uint32_t rec_cap = 0;
+	uint32_t *d_nbr = nullptr;
+	Counters *d_ctr = nullptr;
+	hcm_mesh_rec *d_recs = nullptr;
+	hcm_job_result *d_res = nullptr;
+	hipEvent_t ev0 = nullptr, ev1 = nullptr;
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t graph_exec = nullptr;
+	uint32_t graph_threads = 0, graph_grid = 0;
+	uint32_t last_mode = 0;
+	bool ran = false;
+};
+
+extern "C" int hcm_last_hip_error(void) { return g_last_hip; }
+
+extern "C" int hcm_device_ok(void) {
+	int n = 0;
+	if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+		return 0;
+	int dev = 0;
+	if (hipGetDevice(&dev) != hipSuccess)
+		return 0;
+	hipDeviceProp_t prop;
+	if (hipGetDeviceProperties(&prop, dev) != hipSuccess)
+		return 0;
+	return std::strncmp(prop.gcnArchName, "gfx950", 6) == 0 ? 1 : 0;
+}
+
+static void dfree(void *p) {
+	if (p)
+		(void)hipFree(p);
+}
+
+static void free_ctx(hcm_ctx *c) {
+	if (!c)
+		return;
+	dfree(c->d_pid_of);
+	dfree(c->d_rows);
+	dfree(c->d_codes);
+	dfree(c->d_meshes);
+	dfree(c->d_tex_flags);
+	dfree(c->d_verts);
+	dfree(c->d_idx);
+	dfree(c->d_pblocks);
+	dfree(c->d_plight);
+	dfree(c->d_pnbr);
+	dfree(c->d_pctr);
+	dfree(c->d_pverts);
+	dfree(c->d_pidx);
+	dfree(c->d_precs);
+	dfree(c->d_pres);
+	if (c->h_pblocks)
+		(void)hipHostFree(c->h_pblocks);
+	if (c->h_plight)
+		(void)hipHostFree(c->h_plight);
+	if (c->stream)
+		(void)hipStreamDestroy(c->stream);
+	delete c;
+}
+
+static int create_impl(hcm_ctx *c, const hcm_registry *registry, uint64_t pool_quads) {
+	DeviceTables t;
+	int r = build_tables(registry, &t);
+	if (r != HCM_OK)
+		return r;
+	HCM_HIP(hipGetDevice(&c->device));
+	hipDeviceProp_t prop;
+	HCM_HIP(hipGetDeviceProperties(&prop, c->device));
+	if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+		return HCM_ERR_NO_DEVICE;
+	c->n_cu = prop.multiProcessorCount;
+	HCM_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+
+	HCM_HIP(dmalloc(&c->d_pid_of, t.pid_of.size()));
+	HCM_HIP(dmalloc(&c->d_rows, t.rows.size()));
+	HCM_HIP(dmalloc(&c->d_codes, t.codes.size()));
+	HCM_HIP(dmalloc(&c->d_meshes, t.meshes.size()));
+	HCM_HIP(dmalloc(&c->d_tex_flags, t.tex_flags.size()));
+	HCM_HIP(hipMemcpy(c->d_pid_of, t.pid_of.data(), t.pid_of.size(), hipMemcpyHostToDevice));
+	HCM_HIP(hipMemcpy(c->d_rows, t.rows.data(), t.rows.size() * sizeof(uint2), hipMemcpyHostToDevice));
+	HCM_HIP(hipMemcpy(c->d_codes, t.codes.data(), t.codes.size() * 4, hipMemcpyHostToDevice));
+	if (!t.meshes.empty())
+		HCM_HIP(hipMemcpy(c->d_meshes, t.meshes.data(), t.meshes.size() * sizeof(hcm_custom_mesh),
+		                  hipMemcpyHostToDevice));
+	if (!t.tex_flags.empty())
+		HCM_HIP(hipMemcpy(c->d_tex_flags, t.tex_flags.data(), t.tex_flags.size(), hipMemcpyHostToDevice));
+	c->reg.pid_of = c->d_pid_of;
+	c->reg.rows = c->d_rows;
+	c->reg.codes = c->d_codes;
+	c->reg.meshes = c->d_meshes;
+	c->reg.tex_flags = c->d_tex_flags;
+	c->reg.n_pid = (uint32_t)t.rows.size();
+	c->reg.n_codes = (uint32_t)t.codes.size();
+	c->reg.n_tex = (uint32_t)t.tex_flags.size();
+	c->reg.flow_pid_base = t.flow_pid_base;
+	c->reg.flow_tex = t.flow_tex;
+	c->reg.src_id = t.src_id;
+	c->reg.flow_id = t.flow_id;
+
+	c->pool_quads = std::min<uint64_t>(pool_quads, 0xffffffffull - (1u << 20));
+	if (c->pool_quads) {
+		HCM_HIP(dmalloc(&c->d_verts, (size_t)c->pool_quads * 4));
+		HCM_HIP(dmalloc(&c->d_idx, (size_t)c->pool_quads * 6));
+	}
+	HCM_HIP(dmalloc(&c->d_pblocks, 27 * HCM_CHUNK_VOXELS));
+	HCM_HIP(dmalloc(&c->d_plight, 27 * HCM_CHUNK_VOXELS));
+	HCM_HIP(dmalloc(&c->d_pnbr, 27));
+	HCM_HIP(dmalloc(&c->d_pctr, 1));
+	HCM_HIP(dmalloc(&c->d_pverts, (size_t)kPlainPoolQuads * 4));
+	HCM_HIP(dmalloc(&c->d_pidx, (size_t)kPlainPoolQuads * 6));
+	HCM_HIP(dmalloc(&c->d_precs, kPlainRecs));
+	HCM_HIP(dmalloc(&c->d_pres, 1));
+	HCM_HIP(hipHostMalloc(reinterpret_cast<void **>(&c->h_pblocks), 27 * HCM_CHUNK_VOXELS * 2));
+	HCM_HIP(hipHostMalloc(reinterpret_cast<void **>(&c->h_plight), 27 * HCM_CHUNK_VOXELS));
+	uint32_t ident[27];
+	for (uint32_t i = 0; i < 27; ++i)
+		ident[i] = i;
+	HCM_HIP(hipMemcpy(c->d_pnbr, ident, sizeof(ident), hipMemcpyHostToDevice));
+	return HCM_OK;
+}
+
+extern "C" int hcm_create(const hcm_registry *registry, uint64_t pool_quads, hcm_ctx **out) {
+	if (!out)
+		return HCM_ERR_INVALID;
+	*out = nullptr;
+	int n = 0;
+	if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+		return HCM_ERR_NO_DEVICE;
+	hcm_ctx *c = new hcm_ctx;
+	int r = create_impl(c, registry, pool_quads);
+	if (r != HCM_OK) {
+		free_ctx(c);
+		return r;
+	}
+	*out = c;
+	return HCM_OK;
+}
+
+extern "C" void hcm_destroy(hcm_ctx *ctx) {
+	if (ctx && ctx->stream)
+		(void)hipStreamSynchronize(ctx->stream);
+	free_ctx(ctx);
+}
+
+extern "C" int hcm_set_tuning(hcm_ctx *ctx, const hcm_tuning *t) {
+	if (!ctx || !t)
+		return HCM_ERR_INVALID;
+	uint32_t th = t->threads ? t->threads : 256;
+	if (th != 128 && th != 256 && th != 512)
+		return HCM_ERR_INVALID;
+	std::lock_guard<std::mutex> lk(ctx->mu);
+	ctx->tuning = *t;
+	ctx->tuning.threads = th;
+	return HCM_OK;
+}
+
+extern "C" int hcm_get_tuning(hcm_ctx *ctx, hcm_tuning *t) {
+	if (!ctx || !t)
+		return HCM_ERR_INVALID;
+	std::lock_guard<std::mutex> lk(ctx->mu);
+	*t = ctx->tuning;
+	return HCM_OK;
+}
+
+static MeshParams base_params(hcm_ctx *c) {
+	MeshParams p;
+	std::memset(&p, 0, sizeof(p));
+	p.reg = c->reg;
+	return p;
+}
+
+// Occupancy-derived persistent grid: workgroups per CU times CUs.
+static int persistent_grid(hcm_ctx *c, uint32_t threads) {
+	uint32_t per_cu = c->tuning.wg_per_cu;
+	if (!per_cu) {
+		int lds = mesh_lds_bytes((int)threads);
+		per_cu = lds > 0 ? std::max(1, (160 * 1024) / lds) : 1;
+		per_cu = std::min<uint32_t>(per_cu, 2048 / threads);
+	}
+	return (int)(per_cu * (uint32_t)c->n_cu);
+}
+
+// Assemble one job's meshes from a device pool into a caller's host buffers.
+static int fetch_job(hipStream_t stream, const hcm_job_result &res, const hcm_mesh_rec *d_recs, const uint2 *d_verts,
+                     const uint16_t *d_idx, hcm_host_result *out) {
+	if (res.status != HCM_OK)
+		return res.status;
+	std::vector<hcm_mesh_rec> recs(res.mesh_count);
+	if (res.mesh_count)
+		HCM_HIP(hipMemcpyAsync(recs.data(), d_recs + res.first_mesh, res.mesh_count * sizeof(hcm_mesh_rec),
+		                       hipMemcpyDeviceToHost, stream));
+	HCM_HIP(hipStreamSynchronize(stream));
+	uint64_t nv = 0, ni = 0;
+	for (auto &m : recs)
+		nv += 4ull * m.quad_count, ni += 6ull * m.quad_count;
+	out->vertex_count = (uint32_t)nv;
+	out->index_count = (uint32_t)ni;
+	out->mesh_count = res.mesh_count;
+	if (nv > out->vertex_capacity || ni > out->index_capacity || res.mesh_count > out->mesh_capacity ||
+	    (nv && (!out->vertices || !out->indices)) || (res.mesh_count && !out->meshes))
+		return HCM_ERR_CAPACITY;
+	uint32_t vb = 0, ib = 0;
+	for (uint32_t m = 0; m < res.mesh_count; ++m) {
+		const hcm_mesh_rec &r = recs[m];
+		hcm_mesh_desc &d = out->meshes[m];
+		d.vert_begin = vb, d.vert_count = 4 * r.quad_count;
+		d.idx_begin = ib, d.idx_count = 6 * r.quad_count;
+		std::memcpy(d.aabb, r.aabb, sizeof(d.aabb));
+		d.transparent = r.transparent;
+		HCM_HIP(hipMemcpyAsync(out->vertices + vb, d_verts + 4ull * r.first_quad, 4ull * r.quad_count * sizeof(uint2),
+		                       hipMemcpyDeviceToHost, stream));
+		HCM_HIP(hipMemcpyAsync(out->indices + ib, d_idx + 6ull * r.first_quad, 6ull * r.quad_count * 2,
+		                       hipMemcpyDeviceToHost, stream));
+		vb += d.vert_count, ib += d.idx_count;
+	}
+	HCM_HIP(hipStreamSynchronize(stream));
+	return HCM_OK;
+}
+
+extern "C" int hcm_mesh_chunk(hcm_ctx *ctx, const uint16_t *const blocks[27], const uint8_t *const light[27],
+                              hcm_host_result *out) {
+	if (!ctx || !blocks || !light || !out)
+		return HCM_ERR_INVALID;
+	for (int i = 0; i < 27; ++i)
+		if (!blocks[i] || !light[i])
+			return HCM_ERR_INVALID;
+	std::lock_guard<std::mutex> lk(ctx->mu);
+	for (int i = 0; i < 27; ++i) {
+		std::memcpy(ctx->h_pblocks + (size_t)i * HCM_CHUNK_VOXELS, blocks[i], HCM_CHUNK_VOXELS * 2);
+		std::memcpy(ctx->h_plight + (size_t)i * HCM_CHUNK_VOXELS, light[i], HCM_CHUNK_VOXELS);
+	}
+	hipStream_t st = ctx->stream;
+	HCM_HIP(hipMemcpyAsync(ctx->d_pblocks, ctx->h_pblocks, 27 * HCM_CHUNK_VOXELS * 2, hipMemcpyHostToDevice, st));
+	HCM_HIP(hipMemcpyAsync(ctx->d_plight, ctx->h_plight, 27 * HCM_CHUNK_VOXELS, hipMemcpyHostToDevice, st));
+	HCM_HIP(launch_reset(ctx->d_pctr, st));
+	MeshParams p = base_params(ctx);
+	p.blocks = ctx->d_pblocks;
+	p.light = ctx->d_plight;
+	p.nbr = ctx->d_pnbr;
+	p.n_jobs = 1;
+	p.job_base = 0;
+	p.persistent = 0;
+	p.pool_cap = kPlainPoolQuads;
+	p.ctr = ctx->d_pctr;
+	p.pool_verts = ctx->d_pverts;
+	p.pool_idx = ctx->d_pidx;
+	p.recs = ctx->d_precs;
+	p.rec_cap = kPlainRecs;
+	p.results = ctx->d_pres;
+	HCM_HIP(launch_mesh(p, (int)ctx->tuning.threads, 1, st));
+	hcm_job_result res;
+	HCM_HIP(hipMemcpyAsync(&res, ctx->d_pres, sizeof(res), hipMemcpyDeviceToHost, st));
+	HCM_HIP(hipStreamSynchronize(st));
+	return fetch_job(st, res, ctx->d_precs, ctx->d_pverts, ctx->d_pidx, out);
+}
+
+// ------------------------------------------------------------------------------------------------
+// store
+// ------------------------------------------------------------------------------------------------
+extern "C" int hcm_store_create(hcm_ctx *ctx, uint32_t n_slots, hcm_store **out) {
+	if (!ctx || !out || n_slots == 0)
+		return HCM_ERR_INVALID;
+	*out = nullptr;
+	hcm_store *s = new hcm_store;
+	s->ctx = ctx;
+	s->n_slots = n_slots;
+	if (dmalloc(&s->d_blocks, (size_t)n_slots * HCM_CHUNK_VOXELS) != hipSuccess ||
+	    dmalloc(&s->d_light, (size_t)n_slots * HCM_CHUNK_VOXELS) != hipSuccess) {
+		g_last_hip = (int)hipGetLastError();
+		dfree(s->d_blocks);
+		dfree(s->d_light);
+		delete s;
+		return HCM_ERR_HIP;
+	}
+	*out = s;
+	return HCM_OK;
+}
+
+extern "C" void hcm_store_destroy(hcm_store *s) {
+	if (!s)
+		return;
+	(void)hipStreamSynchronize(s->ctx->stream);
+	dfree(s->d_blocks);
+	dfree(s->d_light);
+	delete s;
+}
+
+extern "C" int hcm_store_upload(hcm_store *s, uint32_t first, uint32_t n, const uint16_t *blocks, const uint8_t *light) {
+	if (!s || !blocks || !light || (uint64_t)first + n > s->n_slots)
+		return HCM_ERR_INVALID;
+	hipStream_t st = s->ctx->stream;
+	HCM_HIP(hipMemcpyAsync(s->d_blocks + (size_t)first * HCM_CHUNK_VOXELS, blocks, (size_t)n * HCM_CHUNK_VOXELS * 2,
+	                       hipMemcpyHostToDevice, st));
+	HCM_HIP(hipMemcpyAsync(s->d_light + (size_t)first * HCM_CHUNK_VOXELS, light, (size_t)n * HCM_CHUNK_VOXELS,
+	                       hipMemcpyHostToDevice, st));
+	HCM_HIP(hipStreamSynchronize(st));
+	return HCM_OK;
+}
+
+extern "C" int hcm_store_device_ptrs(hcm_store *s, void **blocks, void **light) {
+	if (!s)
+		return HCM_ERR_INVALID;
+	if (blocks)
+		*blocks = s->d_blocks;
+	if (light)
+		*light = s->d_light;
+	return HCM_OK;
+}
+
+// ------------------------------------------------------------------------------------------------
+// batch
+// ------------------------------------------------------------------------------------------------
+extern "C" int hcm_batch_create(hcm_ctx *ctx, hcm_store *store, const uint32_t *nbr, uint32_t n_jobs,
+                                hcm_batch **out) {
+	if (!ctx || !store || !nbr || !out || n_jobs == 0 || store->ctx != ctx)
+		return HCM_ERR_INVALID;
+	*out = nullptr;
+	for (uint64_t i = 0; i < 27ull * n_jobs; ++i)
+		if (nbr[i] >= store->n_slots)
+			return HCM_ERR_INVALID;
+	hcm_batch *b = new hcm_batch;
+	b->ctx = ctx;
+	b->store = store;
+	b->n_jobs = n_jobs;
+	b->rec_cap = 3 * n_jobs + 1024;
+	auto fail = [&]() {
+		g_last_hip = (int)hipGetLastError();
+		hcm_batch_destroy(b);
+		return HCM_ERR_HIP;
+	};
+	if (dmalloc(&b->d_nbr, 27ull * n_jobs) != hipSuccess || dmalloc(&b->d_ctr, 1) != hipSuccess ||
+	    dmalloc(&b->d_recs, b->rec_cap) != hipSuccess || dmalloc(&b->d_res, n_jobs) != hipSuccess ||
+	    hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess)
+		return fail();
+	if (hipMemcpy(b->d_nbr, nbr, 27ull * n_jobs * 4, hipMemcpyHostToDevice) != hipSuccess)
+		return fail();
+	*out = b;
+	return HCM_OK;
+}
+
+extern "C" void hcm_batch_destroy(hcm_batch *b) {
+	if (!b)
+		return;
+	if (b->ctx && b->ctx->stream)
+		(void)hipStreamSynchronize(b->ctx->stream);
+	if (b->graph_exec)
+		(void)hipGraphExecDestroy(b->graph_exec);
+	if (b->graph)
+		(void)hipGraphDestroy(b->graph);
+	if (b->ev0)
+		(void)hipEventDestroy(b->ev0);
+	if (b->ev1)
+		(void)hipEventDestroy(b->ev1);
+	dfree(b->d_nbr);
+	dfree(b->d_ctr);
+	dfree(b->d_recs);
+	dfree(b->d_res);
+	delete b;
+}
+
+static MeshParams batch_params(hcm_batch *b) {
+	hcm_ctx *c = b->ctx;
+	MeshParams p = base_params(c);
+	p.blocks = b->store->d_blocks;
+	p.light = b->store->d_light;
+	p.nbr = b->d_nbr;
+	p.n_jobs = b->n_jobs;
+	p.pool_cap = (uint32_t)c->pool_quads;
+	p.ctr = b->d_ctr;
+	p.pool_verts = c->d_verts;
+	p.pool_idx = c->d_idx;
+	p.recs = b->d_recs;
+	p.rec_cap = b->rec_cap;
+	p.results = b->d_res;
+	return p;
+}
+
+extern "C" int hcm_batch_run(hcm_batch *b, uint32_t mode) {
+	if (!b || (mode != HCM_RUN_PERSISTENT && mode != HCM_RUN_PER_CHUNK))
+		return HCM_ERR_INVALID;
+	hcm_ctx *c = b->ctx;
+	std::lock_guard<std::mutex> lk(c->mu);
+	if (!c->pool_quads)
+		return HCM_ERR_POOL_FULL;
+	hipStream_t st = c->stream;
+	const uint32_t threads = c->tuning.threads;
+	MeshParams p = batch_params(b);
+	if (mode == HCM_RUN_PERSISTENT) {
+		const uint32_t grid = (uint32_t)persistent_grid(c, threads);
+		if (!b->graph_exec || b->graph_threads != threads || b->graph_grid != grid) {
+			if (b->graph_exec)
+				(void)hipGraphExecDestroy(b->graph_exec), b->graph_exec = nullptr;
+			if (b->graph)
+				(void)hipGraphDestroy(b->graph), b->graph = nullptr;
+			p.persistent = 1;
+			HCM_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+			hipError_t e1 = launch_reset(b->d_ctr, st);
+			hipError_t e2 = launch_mesh(p, (int)threads, (int)grid, st);
+			hipError_t e3 = hipStreamEndCapture(st, &b->graph);
+			if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
+				g_last_hip = (int)(e1 != hipSuccess ? e1 : (e2 != hipSuccess ? e2 : e3));
+				return HCM_ERR_HIP;
+			}
+			HCM_HIP(hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0));
+			b->graph_threads = threads;
+			b->graph_grid = grid;
+		}
+		HCM_HIP(hipEventRecord(b->ev0, st));
+		HCM_HIP(hipGraphLaunch(b->graph_exec, st));
+		HCM_HIP(hipEventRecord(b->ev1, st));
+	} else {
+		HCM_HIP(launch_reset(b->d_ctr, st));
+		p.persistent = 0;
+		HCM_HIP(hipEventRecord(b->ev0, st));
+		for (uint32_t j = 0; j < b->n_jobs; ++j) {
+			p.job_base = j;
+			HCM_HIP(launch_mesh(p, (int)threads, 1, st));
+		}
+		HCM_HIP(hipEventRecord(b->ev1, st));
+	}
+	b->ran = true;
+	b->last_mode = mode;
+	return HCM_OK;
+}
+
+extern "C" int hcm_batch_sync(hcm_batch *b, float *kernel_ms) {
+	if (!b || !b->ran)
+		return HCM_ERR_INVALID;
+	HCM_HIP(hipEventSynchronize(b->ev1));
+	if (kernel_ms)
+		HCM_HIP(hipEventElapsedTime(kernel_ms, b->ev0, b->ev1));
+	return HCM_OK;
+}
+
+extern "C" int hcm_batch_results(hcm_batch *b, hcm_job_result *results) {
+	if (!b || !results || !b->ran)
+		return HCM_ERR_INVALID;
+	hipStream_t st = b->ctx->stream;
+	HCM_HIP(hipMemcpyAsync(results, b->d_res, (size_t)b->n_jobs * sizeof(hcm_job_result), hipMemcpyDeviceToHost, st));
+	HCM_HIP(hipStreamSynchronize(st));
+	return HCM_OK;
+}
+
+extern "C" int hcm_batch_totals(hcm_batch *b, uint64_t *quads, uint64_t *meshes) {
+	if (!b || !b->ran)
+		return HCM_ERR_INVALID;
+	Counters ctr;
+	hipStream_t st = b->ctx->stream;
+	HCM_HIP(hipMemcpyAsync(&ctr, b->d_ctr, sizeof(ctr), hipMemcpyDeviceToHost, st));
+	HCM_HIP(hipStreamSynchronize(st));
+	if (quads)
+		*quads = ctr.pool_cursor;
+	if (meshes)
+		*meshes = ctr.mesh_cursor;
+	return HCM_OK;
+}
+
+extern "C" int hcm_batch_fetch(hcm_batch *b, uint32_t job, hcm_host_result *out) {
+	if (!b || !out || job >= b->n_jobs || !b->ran)
+		return HCM_ERR_INVALID;
+	hipStream_t st = b->ctx->stream;
+	hcm_job_result res;
+	HCM_HIP(hipMemcpyAsync(&res, b->d_res + job, sizeof(res), hipMemcpyDeviceToHost, st));
+	HCM_HIP(hipStreamSynchronize(st));
+	return fetch_job(st, res, b->d_recs, b->ctx->d_verts, b->ctx->d_idx, out);
+}
+
+extern "C" int hcm_batch_download(hcm_batch *b, hcm_job_result *results, hcm_mesh_rec *recs, uint32_t rec_cap,
+                                  uint32_t (*vertices)[2], uint16_t *indices, uint64_t quad_cap) {
+	if (!b || !b->ran)
+		return HCM_ERR_INVALID;
+	hipStream_t st = b->ctx->stream;
+	Counters ctr;
+	HCM_HIP(hipMemcpyAsync(&ctr, b->d_ctr, sizeof(ctr), hipMemcpyDeviceToHost, st));
+	HCM_HIP(hipStreamSynchronize(st));
+	const uint64_t nq = std::min<uint64_t>(ctr.pool_cursor, b->ctx->pool_quads);
+	const uint32_t nr = std::min<uint32_t>(ctr.mesh_cursor, b->rec_cap);
+	if (results)
+		HCM_HIP(hipMemcpyAsync(results, b->d_res, (size_t)b->n_jobs * sizeof(hcm_job_result), hipMemcpyDeviceToHost,
+		                       st));
+	if (recs) {
+		if (nr > rec_cap)
+			return HCM_ERR_CAPACITY;
+		if (nr)
+			HCM_HIP(hipMemcpyAsync(recs, b->d_recs, (size_t)nr * sizeof(hcm_mesh_rec), hipMemcpyDeviceToHost, st));
+	}
+	if (vertices || indices) {
+		if (nq > quad_cap)
+			return HCM_ERR_CAPACITY;
+		if (nq && vertices)
+			HCM_HIP(hipMemcpyAsync(vertices, b->ctx->d_verts, (size_t)nq * 4 * sizeof(uint2), hipMemcpyDeviceToHost, st));
+		if (nq && indices)
+			HCM_HIP(hipMemcpyAsync(indices, b->ctx->d_idx, (size_t)nq * 12, hipMemcpyDeviceToHost, st));
+	}
+	HCM_HIP(hipStreamSynchronize(st));
+	return HCM_OK;
+}

@@ -1,0 +1,373 @@
+"""ctypes bindings for libhcm.so (the C ABI in include/hcm.h).
+
+Python is plumbing here: it drives the C ABI from the tests and the benchmark. The product
+is libhcm.so, whose meshing runs only in the gfx950 kernels; if the library is missing or no
+gfx950 device is visible, every entry point that meshes raises instead of falling back.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import build as _build
+
+LIB_PATH = _build.LIB
+
+HCM_OK = 0
+HCM_ERR_INVALID = -1
+HCM_ERR_HIP = -2
+HCM_ERR_POOL_FULL = -3
+HCM_ERR_CAPACITY = -4
+HCM_ERR_NO_DEVICE = -5
+HCM_ERR_BUSY = -6
+
+RUN_PERSISTENT = 0
+RUN_PER_CHUNK = 1
+
+HCM_GEN_PLANTS = 1
+
+
+class MeshVertexC(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_uint8), ("y", ctypes.c_uint8), ("z", ctypes.c_uint8), ("ao", ctypes.c_uint8)]
+
+
+class MeshFaceC(ctypes.Structure):
+    _fields_ = [("axis", ctypes.c_uint8), ("light_face", ctypes.c_uint8), ("render_face", ctypes.c_uint8),
+                ("pad", ctypes.c_uint8), ("texture", ctypes.c_uint16), ("pad2", ctypes.c_uint16),
+                ("v", MeshVertexC * 4)]
+
+
+class CustomMeshC(ctypes.Structure):
+    _fields_ = [("face_count", ctypes.c_uint32), ("kind", ctypes.c_uint32), ("flow_height", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32), ("faces", MeshFaceC * 16)]
+
+
+class BlockRowC(ctypes.Structure):
+    _fields_ = [("tex", ctypes.c_uint16 * 6), ("transparency", ctypes.c_uint8), ("collision", ctypes.c_uint8),
+                ("mesh", ctypes.c_uint16)]
+
+
+class RegistryC(ctypes.Structure):
+    _fields_ = [("n_rows", ctypes.c_uint32), ("n_meshes", ctypes.c_uint32), ("n_tex", ctypes.c_uint32),
+                ("row_of", ctypes.POINTER(ctypes.c_uint16)), ("rows", ctypes.POINTER(BlockRowC)),
+                ("meshes", ctypes.POINTER(CustomMeshC)), ("tex_flags", ctypes.POINTER(ctypes.c_uint8)),
+                ("liquid_source_id", ctypes.c_uint8), ("liquid_flow_id", ctypes.c_uint8),
+                ("liquid_flow_tex", ctypes.c_uint16)]
+
+
+class MeshDescC(ctypes.Structure):
+    _fields_ = [("vert_begin", ctypes.c_uint32), ("vert_count", ctypes.c_uint32), ("idx_begin", ctypes.c_uint32),
+                ("idx_count", ctypes.c_uint32), ("aabb", ctypes.c_uint32 * 6), ("transparent", ctypes.c_uint32)]
+
+
+class HostResultC(ctypes.Structure):
+    _fields_ = [("vertices", ctypes.c_void_p), ("vertex_capacity", ctypes.c_uint32),
+                ("indices", ctypes.c_void_p), ("index_capacity", ctypes.c_uint32),
+                ("meshes", ctypes.POINTER(MeshDescC)), ("mesh_capacity", ctypes.c_uint32),
+                ("vertex_count", ctypes.c_uint32), ("index_count", ctypes.c_uint32), ("mesh_count", ctypes.c_uint32)]
+
+
+class JobResultC(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("first_mesh", ctypes.c_uint32), ("mesh_count", ctypes.c_uint32),
+                ("quad_count", ctypes.c_uint32)]
+
+
+class TuningC(ctypes.Structure):
+    _fields_ = [("wg_per_cu", ctypes.c_uint32), ("threads", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 2)]
+
+
+JOB_RESULT_DTYPE = np.dtype([("status", "<i4"), ("first_mesh", "<u4"), ("mesh_count", "<u4"), ("quad_count", "<u4")])
+MESH_REC_DTYPE = np.dtype([("aabb", "<u4", (6,)), ("first_quad", "<u4"), ("quad_count", "<u4"), ("job", "<u4"),
+                           ("transparent", "<u4")])
+
+# (name, restype, argtypes) for every entry point of include/hcm.h and include/hcm_workload.h
+_VP = ctypes.c_void_p
+SIGNATURES = [
+    ("hcm_registry_default", ctypes.POINTER(RegistryC), []),
+    ("hcm_create", ctypes.c_int, [_VP, ctypes.c_uint64, ctypes.POINTER(_VP)]),
+    ("hcm_destroy", None, [_VP]),
+    ("hcm_last_hip_error", ctypes.c_int, []),
+    ("hcm_device_ok", ctypes.c_int, []),
+    ("hcm_mesh_chunk", ctypes.c_int, [_VP, _VP, _VP, ctypes.POINTER(HostResultC)]),
+    ("hcm_store_create", ctypes.c_int, [_VP, ctypes.c_uint32, ctypes.POINTER(_VP)]),
+    ("hcm_store_destroy", None, [_VP]),
+    ("hcm_store_upload", ctypes.c_int, [_VP, ctypes.c_uint32, ctypes.c_uint32, _VP, _VP]),
+    ("hcm_store_device_ptrs", ctypes.c_int, [_VP, ctypes.POINTER(_VP), ctypes.POINTER(_VP)]),
+    ("hcm_batch_create", ctypes.c_int, [_VP, _VP, _VP, ctypes.c_uint32, ctypes.POINTER(_VP)]),
+    ("hcm_batch_destroy", None, [_VP]),
+    ("hcm_batch_run", ctypes.c_int, [_VP, ctypes.c_uint32]),
+    ("hcm_batch_sync", ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_float)]),
+    ("hcm_batch_results", ctypes.c_int, [_VP, _VP]),
+    ("hcm_batch_fetch", ctypes.c_int, [_VP, ctypes.c_uint32, ctypes.POINTER(HostResultC)]),
+    ("hcm_batch_totals", ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]),
+    ("hcm_batch_download", ctypes.c_int, [_VP, _VP, _VP, ctypes.c_uint32, _VP, _VP, ctypes.c_uint64]),
+    ("hcm_set_tuning", ctypes.c_int, [_VP, ctypes.POINTER(TuningC)]),
+    ("hcm_get_tuning", ctypes.c_int, [_VP, ctypes.POINTER(TuningC)]),
+    ("hcm_gen_terrain_chunk", None, [ctypes.c_uint64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                     ctypes.c_uint32, _VP, _VP]),
+    ("hcm_gen_cave_chunk", None, [ctypes.c_uint64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                  ctypes.c_uint32, _VP, _VP]),
+    ("hcm_gen_random_chunk", None, [ctypes.c_uint64, ctypes.c_uint64, _VP, _VP]),
+    ("hcm_gen_opaque_values", ctypes.c_uint32, [_VP, ctypes.c_uint32]),
+]
+
+_lib = None
+
+
+def lib():
+    """Loads libhcm.so (built by hypercraft_amd/build.py). Raises if it was not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+        l = ctypes.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            fn = getattr(l, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = l
+    return _lib
+
+
+class HcmError(RuntimeError):
+    def __init__(self, what, code):
+        hip = lib().hcm_last_hip_error() if code == HCM_ERR_HIP else 0
+        super().__init__(f"{what} failed: status {code}" + (f" (hip error {hip})" if hip else ""))
+        self.code = code
+
+
+def check(what, code):
+    if code != HCM_OK:
+        raise HcmError(what, code)
+
+
+def device_ok():
+    return bool(lib().hcm_device_ok())
+
+
+def require_device():
+    if not device_ok():
+        raise RuntimeError("no gfx950 device visible: the HIP mesher has no CPU fallback")
+
+
+class Mesh:
+    """One BlockMesh: vertices (n, 2) u32, indices (m,) u16, aabb (6-tuple), transparent flag."""
+
+    __slots__ = ("vertices", "indices", "aabb", "transparent")
+
+    def __init__(self, vertices, indices, aabb, transparent):
+        self.vertices = vertices
+        self.indices = indices
+        self.aabb = aabb
+        self.transparent = transparent
+
+
+def _ptrs27(arrs, dtype):
+    arrs = [np.ascontiguousarray(a, dtype=dtype).reshape(-1) for a in arrs]
+    assert len(arrs) == 27 and all(a.size == 4096 for a in arrs)
+    p = (ctypes.c_void_p * 27)(*[a.ctypes.data for a in arrs])
+    return p, arrs
+
+
+class _HostOut:
+    def __init__(self, vcap=1 << 19, mcap=64):
+        self.v = np.zeros((vcap, 2), np.uint32)
+        self.i = np.zeros(vcap * 2, np.uint16)
+        self.m = (MeshDescC * mcap)()
+        self.c = HostResultC(self.v.ctypes.data, vcap, self.i.ctypes.data, vcap * 2,
+                             ctypes.cast(self.m, ctypes.POINTER(MeshDescC)), mcap, 0, 0, 0)
+
+    def meshes(self):
+        out = []
+        for k in range(self.c.mesh_count):
+            d = self.m[k]
+            out.append(Mesh(self.v[d.vert_begin:d.vert_begin + d.vert_count].copy(),
+                            self.i[d.idx_begin:d.idx_begin + d.idx_count].copy(),
+                            tuple(int(a) for a in d.aabb), int(d.transparent)))
+        return out
+
+
+class Context:
+    """An hcm_ctx: registry on the device plus a quad pool of pool_quads quads."""
+
+    def __init__(self, pool_quads=1 << 22, registry=None):
+        require_device()
+        l = lib()
+        self._ptr = ctypes.c_void_p()
+        reg = registry if registry is not None else l.hcm_registry_default()
+        check("hcm_create", l.hcm_create(ctypes.cast(reg, ctypes.c_void_p), pool_quads, ctypes.byref(self._ptr)))
+        self._out = _HostOut()
+
+    @property
+    def ptr(self):
+        return self._ptr
+
+    def close(self):
+        if self._ptr:
+            lib().hcm_destroy(self._ptr)
+            self._ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_tuning(self, wg_per_cu=0, threads=256):
+        t = TuningC(wg_per_cu, threads, (ctypes.c_uint32 * 2)(0, 0))
+        check("hcm_set_tuning", lib().hcm_set_tuning(self._ptr, ctypes.byref(t)))
+
+    def get_tuning(self):
+        t = TuningC()
+        check("hcm_get_tuning", lib().hcm_get_tuning(self._ptr, ctypes.byref(t)))
+        return {"wg_per_cu": t.wg_per_cu, "threads": t.threads}
+
+    def mesh_chunk(self, blocks27, light27):
+        """Plain path: meshes slot 26 of 27 host chunks; returns the reference's list of BlockMesh."""
+        pb, keep_b = _ptrs27(blocks27, np.uint16)
+        pl, keep_l = _ptrs27(light27, np.uint8)
+        check("hcm_mesh_chunk", lib().hcm_mesh_chunk(self._ptr, pb, pl, ctypes.byref(self._out.c)))
+        return self._out.meshes()
+
+
+class Store:
+    """Device-resident chunk slots (hcm_store)."""
+
+    def __init__(self, ctx, n_slots):
+        self.ctx = ctx
+        self._ptr = ctypes.c_void_p()
+        self.n_slots = n_slots
+        check("hcm_store_create", lib().hcm_store_create(ctx.ptr, n_slots, ctypes.byref(self._ptr)))
+
+    @property
+    def ptr(self):
+        return self._ptr
+
+    def upload(self, blocks, light, first=0):
+        blocks = np.ascontiguousarray(blocks, np.uint16).reshape(-1, 4096)
+        light = np.ascontiguousarray(light, np.uint8).reshape(-1, 4096)
+        assert blocks.shape == light.shape
+        check("hcm_store_upload", lib().hcm_store_upload(self._ptr, first, blocks.shape[0], blocks.ctypes.data,
+                                                          light.ctypes.data))
+
+    def close(self):
+        if self._ptr:
+            lib().hcm_store_destroy(self._ptr)
+            self._ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """A set of chunk jobs (27 slot indices each) over a Store (hcm_batch)."""
+
+    def __init__(self, ctx, store, nbr):
+        self.ctx = ctx
+        self.store = store
+        nbr = np.ascontiguousarray(nbr, np.uint32).reshape(-1, 27)
+        self.n_jobs = nbr.shape[0]
+        self._ptr = ctypes.c_void_p()
+        check("hcm_batch_create", lib().hcm_batch_create(ctx.ptr, store.ptr, nbr.ctypes.data, self.n_jobs,
+                                                          ctypes.byref(self._ptr)))
+
+    def run(self, mode=RUN_PERSISTENT):
+        check("hcm_batch_run", lib().hcm_batch_run(self._ptr, mode))
+
+    def sync(self):
+        ms = ctypes.c_float()
+        check("hcm_batch_sync", lib().hcm_batch_sync(self._ptr, ctypes.byref(ms)))
+        return ms.value
+
+    def results(self):
+        r = np.zeros(self.n_jobs, JOB_RESULT_DTYPE)
+        check("hcm_batch_results", lib().hcm_batch_results(self._ptr, r.ctypes.data))
+        return r
+
+    def totals(self):
+        q, m = ctypes.c_uint64(), ctypes.c_uint64()
+        check("hcm_batch_totals", lib().hcm_batch_totals(self._ptr, ctypes.byref(q), ctypes.byref(m)))
+        return q.value, m.value
+
+    def fetch(self, job, out=None):
+        out = out or _HostOut()
+        check("hcm_batch_fetch", lib().hcm_batch_fetch(self._ptr, job, ctypes.byref(out.c)))
+        return out.meshes()
+
+    def download(self):
+        """Everything of the last run: (results, mesh records, vertices (Q*4, 2) u32, indices (Q*6,) u16)."""
+        quads, meshes = self.totals()
+        res = np.zeros(self.n_jobs, JOB_RESULT_DTYPE)
+        recs = np.zeros(max(meshes, 1), MESH_REC_DTYPE)
+        verts = np.zeros((max(quads, 1) * 4, 2), np.uint32)
+        idx = np.zeros(max(quads, 1) * 6, np.uint16)
+        check("hcm_batch_download", lib().hcm_batch_download(self._ptr, res.ctypes.data, recs.ctypes.data,
+                                                              recs.shape[0], verts.ctypes.data, idx.ctypes.data,
+                                                              max(quads, 1)))
+        return res, recs[:meshes], verts[:quads * 4], idx[:quads * 6]
+
+    def close(self):
+        if self._ptr:
+            lib().hcm_batch_destroy(self._ptr)
+            self._ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def job_meshes(job, res, recs, verts, idx):
+    """The meshes of one job from a download(): list of Mesh in the reference's order."""
+    r = res[job]
+    if r["status"] != HCM_OK:
+        raise HcmError(f"job {job}", int(r["status"]))
+    out = []
+    for m in recs[r["first_mesh"]:r["first_mesh"] + r["mesh_count"]]:
+        q0, n = int(m["first_quad"]), int(m["quad_count"])
+        out.append(Mesh(verts[4 * q0:4 * (q0 + n)], idx[6 * q0:6 * (q0 + n)],
+                        tuple(int(a) for a in m["aabb"]), int(m["transparent"])))
+    return out
+
+
+# ---- workload generators (include/hcm_workload.h) ----
+def gen_terrain_chunk(seed, cx, cy, cz, flags=0):
+    b = np.zeros(4096, np.uint16)
+    l = np.zeros(4096, np.uint8)
+    lib().hcm_gen_terrain_chunk(seed, cx, cy, cz, flags, b.ctypes.data, l.ctypes.data)
+    return b, l
+
+
+def gen_cave_chunk(seed, cx, cy, cz, flags=0):
+    b = np.zeros(4096, np.uint16)
+    l = np.zeros(4096, np.uint8)
+    lib().hcm_gen_cave_chunk(seed, cx, cy, cz, flags, b.ctypes.data, l.ctypes.data)
+    return b, l
+
+
+def gen_random_chunk(seed, index):
+    b = np.zeros(4096, np.uint16)
+    l = np.zeros(4096, np.uint8)
+    lib().hcm_gen_random_chunk(seed, index, b.ctypes.data, l.ctypes.data)
+    return b, l
+
+
+def opaque_values():
+    out = np.zeros(64, np.uint16)
+    n = lib().hcm_gen_opaque_values(out.ctypes.data, 64)
+    return out[:n]
+
+
+def registry_arrays():
+    """The built-in registry as numpy arrays (for checks against the exported reference table)."""
+    r = lib().hcm_registry_default().contents
+    row_of = np.ctypeslib.as_array(r.row_of, shape=(65536,)).copy()
+    rows = np.frombuffer(ctypes.string_at(r.rows, 16 * r.n_rows), np.uint8).reshape(r.n_rows, 16).copy()
+    meshes = np.frombuffer(ctypes.string_at(r.meshes, 400 * r.n_meshes), np.uint8).reshape(r.n_meshes, 400).copy()
+    tex_flags = np.frombuffer(ctypes.string_at(r.tex_flags, r.n_tex), np.uint8).copy()
+    return {"row_of": row_of, "rows": rows, "meshes": meshes, "tex_flags": tex_flags,
+            "liquid": (r.liquid_source_id, r.liquid_flow_id, r.liquid_flow_tex)}

@@ -1,0 +1,204 @@
+/*
+ * hcm.h -- C ABI of the MI355X (gfx950) chunk mesher for HyperCraft.
+ *
+ * Drop-in for the client's chunk-meshing call
+ *     BlockMeshAlgo<BlockAlgoConfig<InnerPos1, {0,0,0,S,S,S}, kBlockAlgoSwizzleYZX>>{}.Generate(get_block, get_light)
+ * at client/src/ChunkMeshTask.cpp:113-122 (client/include/client/BlockMeshAlgo.hpp:494-521), for S = 16.
+ *
+ * Conventions shared by every entry point:
+ *   - A chunk is 16^3 voxels indexed x + (y*16 + z)*16 (common/include/common/Position.hpp:98-100).
+ *   - A block is Block::m_data: id | meta << 8 (block/include/block/Block.hpp:24-33).
+ *   - A light byte is sunlight | torchlight << 4 (block/include/block/Light.hpp:18-23).
+ *   - The 27 chunks around a meshed chunk are passed in neighbour-slot order
+ *     slot = 9*L[cx] + 3*L[cy] + L[cz] with L = {-1 -> 1, 0 -> 2, +1 -> 0}; the chunk itself is slot 26
+ *     (Position.hpp:138-145, client/src/ChunkMeshTask.cpp:19-34).
+ *   - Output vertices are the reference's packed BlockVertex words
+ *     (client/include/client/BlockVertex.hpp:8-18), indices are u16 relative to their mesh, and meshes
+ *     come in the reference's order: split meshes first, then opaque, then transparent
+ *     (BlockMeshAlgo.hpp:515-520).
+ *   - Every call returns an int status (HCM_OK or a negative HCM_ERR_*); nothing throws across the ABI.
+ *     The reference has no error channel; statuses replace its silent behaviours.
+ */
+#ifndef HCM_H
+#define HCM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HCM_OK 0
+#define HCM_ERR_INVALID (-1)     /* bad argument or registry */
+#define HCM_ERR_HIP (-2)         /* a HIP runtime call failed; hcm_last_hip_error() has the code */
+#define HCM_ERR_POOL_FULL (-3)   /* the device quad pool cannot hold the output (MeshHandle.hpp:65 drops it) */
+#define HCM_ERR_CAPACITY (-4)    /* caller's host buffers too small; the needed counts were written */
+#define HCM_ERR_NO_DEVICE (-5)   /* no gfx950 device, or the library was built without one */
+#define HCM_ERR_BUSY (-6)        /* serving queue full (back-pressure); retry later */
+
+#define HCM_CHUNK 16
+#define HCM_CHUNK_VOXELS 4096
+#define HCM_MAX_FACES 16 /* kBlockMeshMaxFaceCount, block/include/block/BlockMesh.hpp:37 */
+
+/* ------------------------------------------------------------------------------------------------- */
+/* Block registry: the reference's 65,536-entry BlockProperty table (block/generator/Block.cpp.in:30-32)
+ * flattened to distinct rows, with texture flags (texture/internal/generated/block_texture_transparency.inl)
+ * and custom meshes (block/include/block/BlockMesh.hpp:15-48).                                       */
+/* ------------------------------------------------------------------------------------------------- */
+typedef struct hcm_mesh_vertex {
+	uint8_t x, y, z; /* 1/16 block units */
+	uint8_t ao;      /* 4 = interpolated (BlockMesh.hpp:20) */
+} hcm_mesh_vertex;
+
+typedef struct hcm_mesh_face { /* BlockMeshFace, BlockMesh.hpp:21-26 */
+	uint8_t axis, light_face, render_face, pad;
+	uint16_t texture, pad2; /* BlockTexture::m_data = id | transformation << 13 */
+	hcm_mesh_vertex v[4];
+} hcm_mesh_face;
+
+#define HCM_MESH_STATIC 0u
+#define HCM_MESH_LIQUID_FLOW 1u /* flow_mesh_gen, block/block/public/Liquid.hpp:156-302 */
+
+typedef struct hcm_custom_mesh {
+	uint32_t face_count;
+	uint32_t kind;        /* HCM_MESH_STATIC or HCM_MESH_LIQUID_FLOW */
+	uint32_t flow_height; /* liquid flow: get_flow_height(Variant) (Liquid.hpp:151-154) */
+	uint32_t reserved;
+	hcm_mesh_face faces[HCM_MAX_FACES];
+} hcm_custom_mesh;
+
+typedef struct hcm_block_row { /* the BlockProperty fields the mesher reads (BlockProperty.hpp:17-29) */
+	uint16_t tex[6];      /* textures by BlockFace (right, left, top, bottom, front, back) */
+	uint8_t transparency; /* 0 opaque, 1 semi-transparent, 2 transparent */
+	uint8_t collision;    /* 0 none, 1 solid, 2 water */
+	uint16_t mesh;        /* index into meshes; 0xFFFF = none */
+} hcm_block_row;
+
+#define HCM_TEX_TRANSPARENT 1u /* BlockTexture::IsTransparent */
+#define HCM_TEX_TRANS_PASS 2u  /* BlockTexture::UseTransparentPass */
+#define HCM_TEX_LIQUID 4u      /* BlockTexture::IsLiquid */
+#define HCM_NO_MESH 0xFFFFu
+
+typedef struct hcm_registry {
+	uint32_t n_rows, n_meshes, n_tex;
+	const uint16_t *row_of; /* [65536]: Block::m_data -> row */
+	const hcm_block_row *rows;
+	const hcm_custom_mesh *meshes;
+	const uint8_t *tex_flags; /* [n_tex], by texture id */
+	uint8_t liquid_source_id, liquid_flow_id; /* WaterTrait::kSourceID / kFlowID (Water.hpp:6) */
+	uint16_t liquid_flow_tex;                 /* WaterTrait::kFlowTex */
+} hcm_registry;
+
+/* The library's built-in copy of the reference registry (the block headers under block/block/,
+ * block/register/blocks, texture/register/block_textures). A client that already has hc::block::Block
+ * can instead fill an hcm_registry from its own property table. */
+const hcm_registry *hcm_registry_default(void);
+
+/* ------------------------------------------------------------------------------------------------- */
+/* Context                                                                                           */
+/* ------------------------------------------------------------------------------------------------- */
+typedef struct hcm_ctx hcm_ctx;
+
+/* Replaces constructing BlockMeshAlgo (one per worker thread, ChunkTaskPool.hpp:193): uploads the
+ * registry to the current HIP device and allocates a device quad pool of pool_quads quads
+ * (44 bytes each: 4 vertices + 6 indices). The registry is copied. */
+int hcm_create(const hcm_registry *registry, uint64_t pool_quads, hcm_ctx **out);
+void hcm_destroy(hcm_ctx *ctx);
+int hcm_last_hip_error(void);
+/* 1 if a gfx950 device is visible and the kernels are loadable, else 0. */
+int hcm_device_ok(void);
+
+/* ------------------------------------------------------------------------------------------------- */
+/* Plain path: one chunk per call, host buffers (BlockMeshAlgo::Generate)                             */
+/* ------------------------------------------------------------------------------------------------- */
+typedef struct hcm_mesh_desc { /* one BlockMesh (BlockVertex.hpp:20-25) inside hcm_host_result */
+	uint32_t vert_begin, vert_count; /* into vertices (units of vertices) */
+	uint32_t idx_begin, idx_count;   /* into indices */
+	uint32_t aabb[6];                /* AABB<uint32_t> min xyz, max xyz in 1/16 units (dep/AABB.hpp) */
+	uint32_t transparent;
+} hcm_mesh_desc;
+
+typedef struct hcm_host_result {
+	uint32_t (*vertices)[2]; /* caller-owned; 2 words per vertex */
+	uint32_t vertex_capacity;
+	uint16_t *indices;
+	uint32_t index_capacity;
+	hcm_mesh_desc *meshes;
+	uint32_t mesh_capacity;
+	uint32_t vertex_count, index_count, mesh_count; /* written; on HCM_ERR_CAPACITY: the sizes needed */
+} hcm_host_result;
+
+/* Meshes the chunk in slot 26 of blocks/light (27 host arrays of 4096 voxels each), like
+ * Generate(get_block, get_light) over the 27-chunk lambdas of ChunkMeshTask.cpp:117-122.
+ * Thread-safe per context (calls are serialised on the context's stream). */
+int hcm_mesh_chunk(hcm_ctx *ctx, const uint16_t *const blocks[27], const uint8_t *const light[27],
+                   hcm_host_result *out);
+
+/* ------------------------------------------------------------------------------------------------- */
+/* Device-resident chunk store and batched meshing (no counterpart in the reference, which meshes one
+ * chunk per task on CPU threads; this is the batched form of the same call).                        */
+/* ------------------------------------------------------------------------------------------------- */
+typedef struct hcm_store hcm_store;
+
+/* n_slots chunk slots of 4096 u16 blocks + 4096 u8 light bytes in HBM. */
+int hcm_store_create(hcm_ctx *ctx, uint32_t n_slots, hcm_store **out);
+void hcm_store_destroy(hcm_store *store);
+/* Copies n chunks (n*4096 blocks, n*4096 light bytes, host memory) into slots [first, first+n). */
+int hcm_store_upload(hcm_store *store, uint32_t first, uint32_t n, const uint16_t *blocks, const uint8_t *light);
+/* Device pointers of the slot arrays (blocks: n_slots*4096 u16, light: n_slots*4096 u8). */
+int hcm_store_device_ptrs(hcm_store *store, void **blocks, void **light);
+
+typedef struct hcm_mesh_rec { /* device-side mesh record; 40 bytes (cf. ChunkMeshInfo, ChunkMesh.hpp:12-17) */
+	uint32_t aabb[6];
+	uint32_t first_quad; /* vertices at pool_vertices[4*first_quad], indices at pool_indices[6*first_quad] */
+	uint32_t quad_count;
+	uint32_t job;
+	uint32_t transparent;
+} hcm_mesh_rec;
+
+typedef struct hcm_job_result {
+	int32_t status;      /* HCM_OK or HCM_ERR_POOL_FULL */
+	uint32_t first_mesh; /* into the batch's mesh records */
+	uint32_t mesh_count;
+	uint32_t quad_count;
+} hcm_job_result;
+
+typedef struct hcm_batch hcm_batch;
+
+#define HCM_RUN_PERSISTENT 0u /* one persistent launch with a work queue, replayed as a HIP graph */
+#define HCM_RUN_PER_CHUNK 1u  /* one kernel launch per chunk (the plain per-chunk path) */
+
+/* A batch of n_jobs chunk jobs over a store: nbr holds 27 slot indices per job (host array, copied
+ * to the device). Output meshes go to the context's quad pool. */
+int hcm_batch_create(hcm_ctx *ctx, hcm_store *store, const uint32_t *nbr, uint32_t n_jobs, hcm_batch **out);
+void hcm_batch_destroy(hcm_batch *batch);
+/* Enqueues one meshing pass over every job (resets the pool first). Asynchronous. */
+int hcm_batch_run(hcm_batch *batch, uint32_t mode);
+/* Waits for the last run; *kernel_ms receives the HIP-event time of its meshing kernels. */
+int hcm_batch_sync(hcm_batch *batch, float *kernel_ms);
+/* Per-job results of the last run (host array of n_jobs). */
+int hcm_batch_results(hcm_batch *batch, hcm_job_result *results);
+/* Copies one job's meshes from the device pool into host buffers (same layout as hcm_mesh_chunk). */
+int hcm_batch_fetch(hcm_batch *batch, uint32_t job, hcm_host_result *out);
+/* Totals of the last run: quads and mesh records written. */
+int hcm_batch_totals(hcm_batch *batch, uint64_t *quads, uint64_t *meshes);
+/* Copies everything the last run produced: per-job results (n_jobs), mesh records (up to rec_cap) and
+ * the used part of the quad pool (up to quad_cap quads: 4 vertices and 6 indices each). */
+int hcm_batch_download(hcm_batch *batch, hcm_job_result *results, hcm_mesh_rec *recs, uint32_t rec_cap,
+                       uint32_t (*vertices)[2], uint16_t *indices, uint64_t quad_cap);
+
+/* Launch shape of the meshing kernel. 0 = default. */
+typedef struct hcm_tuning {
+	uint32_t wg_per_cu; /* persistent workgroups per compute unit */
+	uint32_t threads;   /* workgroup size: 128, 256 or 512 */
+	uint32_t reserved[2];
+} hcm_tuning;
+int hcm_set_tuning(hcm_ctx *ctx, const hcm_tuning *t);
+int hcm_get_tuning(hcm_ctx *ctx, hcm_tuning *t);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HCM_H */

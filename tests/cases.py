@@ -1,0 +1,100 @@
+"""Deterministic special-case chunk families for parity tests (test helper).
+
+Voxels are drawn with a vectorised splitmix64 of (seed, chunk, voxel), so the inputs are the
+same on every host and numpy version. Each family returns (blocks (27*n, 4096), light, nbr (n, 27)):
+every job owns its 27 chunks.
+"""
+import numpy as np
+
+M64 = (1 << 64) - 1
+
+
+def _mix(x):
+    x = (x + np.uint64(0x9E3779B97F4A7C15)) & np.uint64(M64)
+    x = ((x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)) & np.uint64(M64)
+    x = ((x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)) & np.uint64(M64)
+    return x ^ (x >> np.uint64(31))
+
+
+def stream(seed, n_chunks, salt=0):
+    idx = np.arange(n_chunks * 4096, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        return _mix(idx ^ _mix(np.uint64(seed) * np.uint64(0x1000193) + np.uint64(salt)))
+
+
+# every registered id with a spread of metas (ids 0-24), plus unregistered ids
+ALL_VALUES = np.array([i | (m << 8) for i in range(27) for m in (0, 1, 2, 3, 4, 5, 7, 8, 9, 12, 17, 130, 255)],
+                      np.uint16)
+CUSTOM = np.array([0, 0, 0, 0, 5, 0x105, 0x205, 0x305, 7, 19, 20, 21, 0x121, 0x221, 0x321, 0x421, 0x521, 22, 23,
+                   14, 15, 0x115, 0x415, 0x815, 0x715, 4, 1, 9], np.uint16)
+GLASSY = np.array([0, 0, 9, 12, 14, 0x10, 0x110, 0x210, 0x310, 0x410, 1, 4, 15, 0x315], np.uint16)
+LOGS = np.array([0, 0, 0, 17, 0x117, 0x817, 0x917, 0x1017, 0x1417, 0x1817, 18, 0x218, 2, 24, 13], np.uint16)
+FLOW = np.array([0, 0, 14, 15, 0x115, 0x215, 0x315, 0x415, 0x515, 0x615, 0x715, 0x815, 0x915, 0xF15, 0x6415,
+                 0x8015, 1, 3], np.uint16)
+
+FAMILIES = {"all_values": ALL_VALUES, "custom": CUSTOM, "glassy": GLASSY, "logs": LOGS, "flow": FLOW}
+# palettes without liquid-flow cells (their faces use per-cell dynamic textures)
+STILL = {"glassy_still": np.array([v for v in GLASSY if v & 0xFF != 15], np.uint16)}
+
+
+def palette(name, seed, n_jobs, density=0.6):
+    pal = FAMILIES[name] if name in FAMILIES else STILL[name]
+    r = stream(seed, 27 * n_jobs, salt=hash_name(name))
+    pick = (r >> np.uint64(8)) % np.uint64(len(pal))
+    keep = (r & np.uint64(0xFF)) < np.uint64(int(density * 256))
+    blocks = np.where(keep, pal[pick.astype(np.int64)], 0).astype(np.uint16).reshape(27 * n_jobs, 4096)
+    light = ((r >> np.uint64(40)) & np.uint64(0xFF)).astype(np.uint8).reshape(27 * n_jobs, 4096)
+    nbr = np.arange(27 * n_jobs, dtype=np.uint32).reshape(n_jobs, 27)
+    return blocks, light, nbr
+
+
+def hash_name(name):
+    h = 0
+    for c in name.encode():
+        h = (h * 131 + c) & 0xFFFFFFFF
+    return h
+
+
+def chunk_of(fn):
+    a = np.zeros(4096, np.uint16)
+    for y in range(16):
+        for z in range(16):
+            for x in range(16):
+                a[x + 16 * (z + 16 * y)] = fn(x, y, z)
+    return a
+
+
+def single(blocks27, light27):
+    b = np.stack([np.asarray(x, np.uint16) for x in blocks27])
+    l = np.stack([np.asarray(x, np.uint8) for x in light27])
+    return b, l, np.arange(27, dtype=np.uint32).reshape(1, 27)
+
+
+def leaf_checkerboard():
+    """Every face visible between differing leaf variants: the transparent pass exceeds 16,383 quads."""
+    leaf = chunk_of(lambda x, y, z: 0x10 | ((((x + y + z) % 2) + 2 * (x % 2)) % 5) << 8)
+    return single([leaf] * 27, [np.full(4096, 0x3F, np.uint8)] * 27)
+
+
+def lone_block(value=1, sun=15):
+    air = np.zeros(4096, np.uint16)
+    c = air.copy()
+    c[5 + 16 * (6 + 16 * 7)] = value  # x=5, y=7, z=6
+    b = [air] * 26 + [c]
+    return single(b, [np.full(4096, sun, np.uint8)] * 27)
+
+
+def slab(value=1, top=8):
+    s = chunk_of(lambda x, y, z: value if y < top else 0)
+    air = np.zeros(4096, np.uint16)
+    # neighbours: same slab sideways, solid below, air above
+    blocks = []
+    for sl in range(27):
+        cy = (1, -1, 0)[(sl // 3) % 3]
+        blocks.append(s if cy == 0 else (np.full(4096, value, np.uint16) if cy < 0 else air))
+    light = [chunk_of(lambda x, y, z: 15 if y >= top else 0).astype(np.uint8)] * 27
+    return single(blocks, light)
+
+
+def uniform(value, light_byte=0x5A):
+    return single([np.full(4096, value, np.uint16)] * 27, [np.full(4096, light_byte, np.uint8)] * 27)

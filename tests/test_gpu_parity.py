@@ -1,0 +1,162 @@
+"""Parity of the gfx950 mesher with the reference, through the C ABI (needs an MI355X).
+
+Bit-exact: every job's mesh list (vertices, indices, AABB, transparent flag, mesh order) must have
+the digest the real reference mesher produced for it (tests/golden/golden.npz), and, for the
+plain-path cases, equal the CPU oracle's output.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import cases
+import meshdigest
+from conftest import GOLDEN
+from hypercraft_amd import hcm, workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def golden():
+    return np.load(os.path.join(GOLDEN, "golden.npz"))
+
+
+_SETS = {}
+
+
+def gset(name):
+    if name not in _SETS:
+        import test_oracle
+
+        _SETS.update(test_oracle.sets())
+    return _SETS[name]
+
+
+def test_device_present():
+    assert hcm.device_ok(), "no gfx950 device: the mesher has no CPU fallback"
+
+
+@pytest.mark.parametrize("name", ["lone_stone", "lone_glass", "slab", "uniform_air", "uniform_stone",
+                                  "uniform_water", "uniform_flow", "config1", "config1_plants",
+                                  "leaf_checkerboard"])
+def test_plain_path_single_chunks(ctx, oracle, golden, name):
+    blocks, light, nbr = gset(name)
+    b27, l27 = workloads.slice_jobs(blocks, light, nbr, 0)
+    say(f"plain path: {name}")
+    got = ctx.mesh_chunk(b27, l27)
+    assert meshdigest.digest(got) == golden[name + "_digest"][0].tobytes()
+    want = oracle.mesh(b27, l27)
+    assert meshdigest.digest(got) == meshdigest.digest(want)
+
+
+def test_plain_path_config1_full_meshes(ctx, golden):
+    blocks, light, nbr = gset("config1")
+    got = ctx.mesh_chunk(*workloads.slice_jobs(blocks, light, nbr, 0))
+    for k, m in enumerate(got):
+        assert np.array_equal(m.vertices, golden[f"config1_mesh{k}_vertices"])
+        assert np.array_equal(m.indices, golden[f"config1_mesh{k}_indices"])
+        assert list(m.aabb) + [m.transparent] == golden[f"config1_mesh{k}_meta"].tolist()
+
+
+@pytest.mark.parametrize("fam", list(cases.FAMILIES))
+def test_plain_path_palettes(ctx, golden, fam):
+    name = "palette_sparse_" + fam
+    blocks, light, nbr = gset(name)
+    for j in range(0, nbr.shape[0], 3):
+        say(f"plain path: {name} job {j}")
+        got = ctx.mesh_chunk(*workloads.slice_jobs(blocks, light, nbr, j))
+        assert meshdigest.digest(got) == golden[name + "_digest"][j].tobytes(), f"{name} job {j}"
+
+
+def run_batch(ctx, blocks, light, nbr, mode=hcm.RUN_PERSISTENT):
+    store = hcm.Store(ctx, blocks.shape[0])
+    store.upload(blocks, light)
+    batch = hcm.Batch(ctx, store, nbr)
+    batch.run(mode)
+    batch.sync()
+    out = batch.download()
+    batch.close()
+    store.close()
+    return out
+
+
+def digests_of(out):
+    res, recs, verts, idx = out
+    n = res.shape[0]
+    d = np.zeros((n, 32), np.uint8)
+    for j in range(n):
+        d[j] = np.frombuffer(meshdigest.digest(hcm.job_meshes(j, res, recs, verts, idx)), np.uint8)
+    return d
+
+
+def assert_digests(name, got, want):
+    bad = np.nonzero((got != want).any(1))[0]
+    assert bad.size == 0, f"{name}: {bad.size} jobs differ, first {bad[:10].tolist()}"
+
+
+@pytest.mark.parametrize("name", ["palette_" + f for f in cases.FAMILIES] + ["config3", "config2"])
+def test_batched_persistent(ctx, golden, name):
+    blocks, light, nbr = gset(name)
+    say(f"batched persistent: {name} ({nbr.shape[0]} jobs)")
+    out = run_batch(ctx, blocks, light, nbr)
+    assert (out[0]["status"] == 0).all()
+    assert np.array_equal(out[0]["quad_count"], golden[name + "_quads"])
+    assert_digests(name, digests_of(out), golden[name + "_digest"])
+
+
+def test_per_chunk_launches_match(ctx, golden):
+    blocks, light, nbr = gset("config2")
+    sub = nbr[::37]
+    say(f"per-chunk launches: config2 subset ({sub.shape[0]} jobs)")
+    out = run_batch(ctx, blocks, light, sub, mode=hcm.RUN_PER_CHUNK)
+    assert_digests("config2 subset", digests_of(out), golden["config2_digest"][::37])
+
+
+@pytest.mark.parametrize("threads", [128, 512])
+def test_workgroup_sizes_agree(ctx, golden, threads):
+    blocks, light, nbr = gset("palette_all_values")
+    ctx.set_tuning(wg_per_cu=0, threads=threads)
+    try:
+        say(f"threads {threads}: palette_all_values")
+        out = run_batch(ctx, blocks, light, nbr)
+    finally:
+        ctx.set_tuning(wg_per_cu=0, threads=256)
+    assert_digests(f"threads {threads}", digests_of(out), golden["palette_all_values_digest"])
+
+
+def test_graph_replays_are_identical(ctx, golden):
+    blocks, light, nbr = gset("palette_custom")
+    store = hcm.Store(ctx, blocks.shape[0])
+    store.upload(blocks, light)
+    batch = hcm.Batch(ctx, store, nbr)
+    try:
+        for rep in range(3):
+            say(f"graph replay {rep}")
+            batch.run()
+            batch.sync()
+            out = batch.download()
+            np.asarray(blocks[:10]).sum()  # host work between replays
+            assert_digests(f"replay {rep}", digests_of(out), golden["palette_custom_digest"])
+    finally:
+        batch.close()
+        store.close()
+
+
+def test_pool_full_is_reported():
+    small = hcm.Context(pool_quads=1000)
+    try:
+        blocks, light, nbr = gset("palette_glassy")
+        say("pool full: palette_glassy on a 1000-quad pool")
+        out = run_batch(small, blocks, light, nbr[:8])
+        st = out[0]["status"]
+        assert (st == hcm.HCM_ERR_POOL_FULL).any()
+        assert set(st.tolist()) <= {0, hcm.HCM_ERR_POOL_FULL}
+    finally:
+        small.close()
