@@ -247,63 +247,69 @@ template <int NT> __device__ __forceinline__ uint64_t block_excl_scan(Smem<NT> &
 // liquid flow (block/block/public/Liquid.hpp:151-302)
 // ------------------------------------------------------------------------------------------------
 struct Flow {
-	uint8_t h[4];
-	uint8_t liq_side; // bit per neighbour 1, 3, 4, 6 (back, left, right, front) that is liquid
+	uint32_t h;        // corner heights h0 | h1 << 8 | h2 << 16 | h3 << 24 (0 1 / 2 3 over x, z)
+	uint32_t liq_side; // bit per neighbour 1, 3, 4, 6 (back, left, right, front) that is a liquid
 };
+
+__device__ __forceinline__ uint32_t flow_h(const Flow &f, int i) { return (f.h >> (8 * i)) & 0xffu; }
 
 __device__ __forceinline__ uint16_t global_block(const MeshParams &p, const uint32_t *slots, int x, int y, int z) {
 	return p.blocks[(size_t)slots[slot27(x, y, z)] * HCM_CHUNK_VOXELS + local16(x, y, z)];
 }
 
-__device__ __forceinline__ Flow flow_heights(const MeshParams &p, const uint32_t *slots, int x, int y, int z, uint32_t nb_height) {
-	const int kdx[8] = {-1, 0, 1, -1, 1, -1, 0, 1}, kdz[8] = {-1, -1, -1, 0, 0, 1, 1, 1};
-	int8_t nhs[8];
-	uint32_t liq = 0;
-#pragma unroll
-	for (int i = 0; i < 8; ++i) {
-		uint16_t b = global_block(p, slots, x + kdx[i], y, z + kdz[i]);
-		uint32_t id = b & 0xffu, meta = b >> 8;
-		int h;
-		if (id == p.reg.src_id)
-			h = 16;
-		else if (id == p.reg.flow_id)
-			h = uint8_t(meta * 2u > 1u ? meta * 2u : 1u); // get_flow_height(meta) as uint8_t
-		else
-			h = id != 0 ? -1 : 0; // WaterTrait::IsObstacle: id != kAir (block/block/public/Water.hpp:13)
-		nhs[i] = int8_t(h);
-		if (id == p.reg.src_id || id == p.reg.flow_id)
-			liq |= 1u << i;
+// get_nei_height (:164-169) as a signed integer: the reference returns it as int8_t, so a flow level's
+// uint8_t height max(meta * 2, 1) wraps to [-128, 127].
+__device__ __forceinline__ int flow_nei_height(uint32_t b, uint32_t src_id, uint32_t flow_id) {
+	const uint32_t id = b & 0xffu, meta = b >> 8;
+	if (id == src_id)
+		return 16;
+	if (id == flow_id) {
+		const uint32_t h = (meta * 2u > 1u ? meta * 2u : 1u) & 0xffu;
+		return h >= 128u ? int(h) - 256 : int(h);
 	}
-	const int8_t nb = int8_t(nb_height);
-	auto vh = [&](int a, int b, int c) -> uint8_t { // get_vh, :181-186
-		int8_t m1 = nb > nhs[a] ? nb : nhs[a], m2 = nhs[b] > nhs[c] ? nhs[b] : nhs[c];
-		uint8_t mx = uint8_t(m1 > m2 ? m1 : m2);
-		if (mx != 16 && int(nb == 0) + int(nhs[a] == 0) + int(nhs[b] == 0) + int(nhs[c] == 0) >= 3)
-			return 0;
-		return mx;
-	};
+	return id != 0 ? -1 : 0; // WaterTrait::IsObstacle: id != kAir (block/block/public/Water.hpp:13)
+}
+
+// get_vh (:181-186): the signed maximum as uint8_t, or 0 when three of the four are empty (height 0)
+__device__ __forceinline__ uint32_t flow_vh(int nb, int a, int b, int c) {
+	const int m = max(max(nb, a), max(b, c));
+	const uint32_t mx = uint32_t(m) & 0xffu;
+	const int zeros = int(nb == 0) + int(a == 0) + int(b == 0) + int(c == 0);
+	return (mx != 16u && zeros >= 3) ? 0u : mx;
+}
+
+__device__ __forceinline__ Flow flow_heights(const MeshParams &p, const uint32_t *slots, int x, int y, int z,
+                                             uint32_t nb_height) {
+	const uint32_t src = p.reg.src_id, fid = p.reg.flow_id;
+	// neighbours (:291-300): 0 (-1,-1) 1 (0,-1) 2 (1,-1) 3 (-1,0) 4 (1,0) 5 (-1,1) 6 (0,1) 7 (1,1) in (x, z)
+	const uint32_t b0 = global_block(p, slots, x - 1, y, z - 1), b1 = global_block(p, slots, x, y, z - 1);
+	const uint32_t b2 = global_block(p, slots, x + 1, y, z - 1), b3 = global_block(p, slots, x - 1, y, z);
+	const uint32_t b4 = global_block(p, slots, x + 1, y, z), b5 = global_block(p, slots, x - 1, y, z + 1);
+	const uint32_t b6 = global_block(p, slots, x, y, z + 1), b7 = global_block(p, slots, x + 1, y, z + 1);
+	const int n0 = flow_nei_height(b0, src, fid), n1 = flow_nei_height(b1, src, fid);
+	const int n2 = flow_nei_height(b2, src, fid), n3 = flow_nei_height(b3, src, fid);
+	const int n4 = flow_nei_height(b4, src, fid), n5 = flow_nei_height(b5, src, fid);
+	const int n6 = flow_nei_height(b6, src, fid), n7 = flow_nei_height(b7, src, fid);
+	const int nb = int(nb_height);
 	Flow f;
-	f.h[0] = vh(0, 1, 3);
-	f.h[1] = vh(1, 2, 4);
-	f.h[2] = vh(3, 5, 6);
-	f.h[3] = vh(4, 6, 7);
-	f.liq_side = uint8_t(((liq >> 1) & 1u) | (((liq >> 3) & 1u) << 1) | (((liq >> 4) & 1u) << 2) | (((liq >> 6) & 1u) << 3));
+	f.h = flow_vh(nb, n0, n1, n3) | flow_vh(nb, n1, n2, n4) << 8 | flow_vh(nb, n3, n5, n6) << 16 |
+	      flow_vh(nb, n4, n6, n7) << 24;
+	auto liq = [&](uint32_t b) { return ((b & 0xffu) == src || (b & 0xffu) == fid) ? 1u : 0u; };
+	f.liq_side = liq(b1) | liq(b3) << 1 | liq(b4) << 2 | liq(b6) << 3;
 	return f;
 }
 
-__device__ __forceinline__ bool flow_all_full(const Flow &f) {
-	return f.h[0] == 16 && f.h[1] == 16 && f.h[2] == 16 && f.h[3] == 16;
-}
+__device__ __forceinline__ bool flow_all_full(const Flow &f) { return f.h == 0x10101010u; }
 
 // Emitted faces of a flow cell, in order: top, left, right, back, front (:193-274). Returns a 5-bit set.
 __device__ __forceinline__ uint32_t flow_face_set(const Flow &f) {
 	if (flow_all_full(f))
 		return 0;
-	const uint8_t *h = f.h;
-	bool lf = h[0] == 16 && h[2] == 16, le = h[0] == 0 && h[2] == 0;
-	bool rf = h[1] == 16 && h[3] == 16, re = h[1] == 0 && h[3] == 0;
-	bool ff = h[2] == 16 && h[3] == 16, fe = h[2] == 0 && h[3] == 0;
-	bool bf = h[0] == 16 && h[1] == 16, be = h[0] == 0 && h[1] == 0;
+	const uint32_t h0 = flow_h(f, 0), h1 = flow_h(f, 1), h2 = flow_h(f, 2), h3 = flow_h(f, 3);
+	const bool lf = h0 == 16 && h2 == 16, le = h0 == 0 && h2 == 0;
+	const bool rf = h1 == 16 && h3 == 16, re = h1 == 0 && h3 == 0;
+	const bool ff = h2 == 16 && h3 == 16, fe = h2 == 0 && h3 == 0;
+	const bool bf = h0 == 16 && h1 == 16, be = h0 == 0 && h1 == 0;
 	uint32_t set = 1u;
 	if (!lf && !le && !(f.liq_side & 2u))
 		set |= 2u;
@@ -321,50 +327,45 @@ __device__ __forceinline__ uint32_t flow_face_set(const Flow &f) {
 __device__ __forceinline__ uint32_t flow_pattern(const Flow &f) {
 	if (flow_all_full(f))
 		return 16u;
-	const uint8_t *h = f.h;
-	return uint32_t(h[1] == 16 && h[3] == 16) | (uint32_t(h[0] == 16 && h[2] == 16) << 1) |
-	       (uint32_t(h[2] == 16 && h[3] == 16) << 2) | (uint32_t(h[0] == 16 && h[1] == 16) << 3);
+	const uint32_t h0 = flow_h(f, 0), h1 = flow_h(f, 1), h2 = flow_h(f, 2), h3 = flow_h(f, 3);
+	return uint32_t(h1 == 16 && h3 == 16) | (uint32_t(h0 == 16 && h2 == 16) << 1) |
+	       (uint32_t(h2 == 16 && h3 == 16) << 2) | (uint32_t(h0 == 16 && h1 == 16) << 3);
 }
 
-// The kind-th face (0 top, 1 left, 2 right, 3 back, 4 front) of a flow cell.
+__device__ __forceinline__ hcm_mesh_vertex fv(uint32_t x, uint32_t y, uint32_t z) {
+	hcm_mesh_vertex v;
+	v.x = uint8_t(x), v.y = uint8_t(y), v.z = uint8_t(z), v.ao = 4;
+	return v;
+}
+
+// The kind-th face (0 top, 1 left, 2 right, 3 back, 4 front) of a flow cell (:198-274).
 __device__ hcm_mesh_face flow_face(const Flow &f, int kind, uint16_t tex) {
-	const uint8_t h0 = f.h[0], h1 = f.h[1], h2 = f.h[2], h3 = f.h[3];
+	const uint32_t h0 = flow_h(f, 0), h1 = flow_h(f, 1), h2 = flow_h(f, 2), h3 = flow_h(f, 3);
 	hcm_mesh_face m;
 	m.pad = 0, m.pad2 = 0, m.texture = tex;
-	uint8_t v[4][3];
 	switch (kind) {
 	case 0:
 		m.axis = 1, m.light_face = 2;
-		v[0][0] = 0, v[0][1] = h0, v[0][2] = 0, v[1][0] = 0, v[1][1] = h2, v[1][2] = 16;
-		v[2][0] = 16, v[2][1] = h3, v[2][2] = 16, v[3][0] = 16, v[3][1] = h1, v[3][2] = 0;
+		m.v[0] = fv(0, h0, 0), m.v[1] = fv(0, h2, 16), m.v[2] = fv(16, h3, 16), m.v[3] = fv(16, h1, 0);
 		break;
 	case 1:
 		m.axis = 0, m.light_face = 1;
-		v[0][0] = 0, v[0][1] = h0, v[0][2] = 0, v[1][0] = 0, v[1][1] = 0, v[1][2] = 0;
-		v[2][0] = 0, v[2][1] = 0, v[2][2] = 16, v[3][0] = 0, v[3][1] = h2, v[3][2] = 16;
+		m.v[0] = fv(0, h0, 0), m.v[1] = fv(0, 0, 0), m.v[2] = fv(0, 0, 16), m.v[3] = fv(0, h2, 16);
 		break;
 	case 2:
 		m.axis = 0, m.light_face = 0;
-		v[0][0] = 16, v[0][1] = 0, v[0][2] = 0, v[1][0] = 16, v[1][1] = h1, v[1][2] = 0;
-		v[2][0] = 16, v[2][1] = h3, v[2][2] = 16, v[3][0] = 16, v[3][1] = 0, v[3][2] = 16;
+		m.v[0] = fv(16, 0, 0), m.v[1] = fv(16, h1, 0), m.v[2] = fv(16, h3, 16), m.v[3] = fv(16, 0, 16);
 		break;
 	case 3:
 		m.axis = 2, m.light_face = 5;
-		v[0][0] = 0, v[0][1] = 0, v[0][2] = 0, v[1][0] = 0, v[1][1] = h0, v[1][2] = 0;
-		v[2][0] = 16, v[2][1] = h1, v[2][2] = 0, v[3][0] = 16, v[3][1] = 0, v[3][2] = 0;
+		m.v[0] = fv(0, 0, 0), m.v[1] = fv(0, h0, 0), m.v[2] = fv(16, h1, 0), m.v[3] = fv(16, 0, 0);
 		break;
 	default:
 		m.axis = 2, m.light_face = 4;
-		v[0][0] = 0, v[0][1] = h2, v[0][2] = 16, v[1][0] = 0, v[1][1] = 0, v[1][2] = 16;
-		v[2][0] = 16, v[2][1] = 0, v[2][2] = 16, v[3][0] = 16, v[3][1] = h3, v[3][2] = 16;
+		m.v[0] = fv(0, h2, 16), m.v[1] = fv(0, 0, 16), m.v[2] = fv(16, 0, 16), m.v[3] = fv(16, h3, 16);
 		break;
 	}
 	m.render_face = m.light_face;
-#pragma unroll
-	for (int i = 0; i < 4; ++i) {
-		m.v[i].x = v[i][0], m.v[i].y = v[i][1], m.v[i].z = v[i][2];
-		m.v[i].ao = 4;
-	}
 	return m;
 }
 

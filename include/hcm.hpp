@@ -1,0 +1,127 @@
+// hcm.hpp -- C++ front end of the gfx950 chunk mesher, in the shape of the reference's mesher.
+//
+// hcm::BlockMeshGPU::Generate(get_block, get_light) takes the same two callables as
+// hc::client::BlockMeshAlgo<Config>::Generate (client/include/client/BlockMeshAlgo.hpp:494-521) for a
+// 16^3 chunk and returns the same std::vector<BlockMesh> (client/include/client/BlockVertex.hpp:8-25):
+// packed vertices, u16 indices, AABB<uint32_t> in 1/16 block units, and the transparent flag, in the
+// reference's mesh order. The callables are evaluated over the 48^3 voxels of the chunk and its 26
+// neighbours (x, y, z in [-16, 32)); the meshing itself runs on the GPU through include/hcm.h.
+#pragma once
+
+#include <array>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "hcm.h"
+
+namespace hcm {
+
+struct BlockVertex { // the reference's BlockVertex words (BlockVertex.hpp:8-18)
+	uint32_t x10_y10_z10_axis2, tex10_trans3_face3_ao2_sl6_tl6;
+};
+static_assert(sizeof(BlockVertex) == 8, "two packed words");
+
+struct AABB { // AABB<uint32_t> (dep/AABB.hpp:9-60)
+	uint32_t min[3], max[3];
+};
+
+struct BlockMesh { // BlockMesh (BlockVertex.hpp:20-25)
+	std::vector<BlockVertex> vertices;
+	std::vector<uint16_t> indices;
+	AABB aabb{};
+	bool transparent{};
+};
+
+class Error : public std::runtime_error {
+public:
+	Error(const char *what, int code) : std::runtime_error(std::string(what) + " failed: " + std::to_string(code)), code(code) {}
+	int code;
+};
+
+class BlockMeshGPU {
+public:
+	explicit BlockMeshGPU(const hcm_registry *registry = hcm_registry_default(), uint64_t pool_quads = 0) {
+		int r = hcm_create(registry, pool_quads, &m_ctx);
+		if (r != HCM_OK)
+			throw Error("hcm_create", r);
+		for (int i = 0; i < 27; ++i) {
+			m_block_ptrs[i] = m_blocks[i].data();
+			m_light_ptrs[i] = m_light[i].data();
+		}
+	}
+	~BlockMeshGPU() { hcm_destroy(m_ctx); }
+	BlockMeshGPU(const BlockMeshGPU &) = delete;
+	BlockMeshGPU &operator=(const BlockMeshGPU &) = delete;
+
+	// Same call shape as BlockMeshAlgo::Generate: get_block(x, y, z) returns something convertible to the
+	// 16-bit Block::m_data (e.g. via .GetData()) and get_light(x, y, z) the 8-bit Light::m_data.
+	template <typename GetBlockFunc, typename GetLightFunc>
+	std::vector<BlockMesh> Generate(GetBlockFunc &&get_block_func, GetLightFunc &&get_light_func) {
+		for (int y = -16; y < 32; ++y)
+			for (int z = -16; z < 32; ++z)
+				for (int x = -16; x < 32; ++x) {
+					const int slot = slot_of(x, y, z);
+					const int local = (x & 15) + 16 * ((z & 15) + 16 * (y & 15));
+					m_blocks[slot][local] = data_of(get_block_func(x, y, z));
+					m_light[slot][local] = (uint8_t)data_of(get_light_func(x, y, z));
+				}
+		return Generate(m_block_ptrs, m_light_ptrs);
+	}
+
+	// The 27 chunks directly, in neighbour-slot order (common/include/common/Position.hpp:138-145);
+	// e.g. neighbour_chunks[i]->GetBlockData() at client/src/ChunkMeshTask.cpp:117-119.
+	std::vector<BlockMesh> Generate(const uint16_t *const blocks[27], const uint8_t *const light[27]) {
+		hcm_host_result out{};
+		for (;;) {
+			out.vertices = reinterpret_cast<uint32_t(*)[2]>(m_verts.data());
+			out.vertex_capacity = (uint32_t)m_verts.size();
+			out.indices = m_idx.data();
+			out.index_capacity = (uint32_t)m_idx.size();
+			out.meshes = m_desc.data();
+			out.mesh_capacity = (uint32_t)m_desc.size();
+			int r = hcm_mesh_chunk(m_ctx, blocks, light, &out);
+			if (r == HCM_ERR_CAPACITY) {
+				m_verts.resize(out.vertex_count);
+				m_idx.resize(out.index_count);
+				m_desc.resize(out.mesh_count);
+				continue;
+			}
+			if (r != HCM_OK)
+				throw Error("hcm_mesh_chunk", r);
+			break;
+		}
+		std::vector<BlockMesh> meshes(out.mesh_count);
+		for (uint32_t m = 0; m < out.mesh_count; ++m) {
+			const hcm_mesh_desc &d = m_desc[m];
+			meshes[m].vertices.assign(m_verts.begin() + d.vert_begin, m_verts.begin() + d.vert_begin + d.vert_count);
+			meshes[m].indices.assign(m_idx.begin() + d.idx_begin, m_idx.begin() + d.idx_begin + d.idx_count);
+			for (int a = 0; a < 3; ++a)
+				meshes[m].aabb.min[a] = d.aabb[a], meshes[m].aabb.max[a] = d.aabb[3 + a];
+			meshes[m].transparent = d.transparent != 0;
+		}
+		return meshes;
+	}
+
+	hcm_ctx *context() const { return m_ctx; }
+
+private:
+	static int slot_of(int x, int y, int z) {
+		auto l = [](int v) { return v < 0 ? 1 : (v >= 16 ? 0 : 2); };
+		return 9 * l(x) + 3 * l(y) + l(z);
+	}
+	template <typename T> static auto data_of(const T &v) -> decltype(v.GetData()) { return v.GetData(); }
+	static uint32_t data_of(uint32_t v) { return v; }
+
+	hcm_ctx *m_ctx = nullptr;
+	std::array<std::array<uint16_t, HCM_CHUNK_VOXELS>, 27> m_blocks{};
+	std::array<std::array<uint8_t, HCM_CHUNK_VOXELS>, 27> m_light{};
+	const uint16_t *m_block_ptrs[27];
+	const uint8_t *m_light_ptrs[27];
+	std::vector<BlockVertex> m_verts = std::vector<BlockVertex>(1 << 16);
+	std::vector<uint16_t> m_idx = std::vector<uint16_t>(3 << 15);
+	std::vector<hcm_mesh_desc> m_desc = std::vector<hcm_mesh_desc>(8);
+};
+
+} // namespace hcm
