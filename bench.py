@@ -66,7 +66,7 @@ def main():
 
     hcm.require_device()
     log("generating the 4096-chunk cave batch")
-    blocks, light, nbr = workloads.config2()
+    blocks, light, nbr = workloads.cave_region()
     golden = np.load(os.path.join(ROOT, "tests", "golden", "golden.npz"))
     ctx = hcm.Context(pool_quads=1 << 23)
     store = hcm.Store(ctx, blocks.shape[0])
@@ -91,7 +91,7 @@ def main():
     bad = 0
     for j in range(n):
         d = meshdigest.digest(hcm.job_meshes(j, *out))
-        bad += d != golden["config2_digest"][j].tobytes()
+        bad += d != golden["cave_region_digest"][j].tobytes()
     bit_exact = bad == 0 and (out[0]["status"] == 0).all()
 
     plain = None

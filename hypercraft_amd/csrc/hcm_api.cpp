@@ -37,6 +37,7 @@ template <typename T> hipError_t dmalloc(T **p, size_t count) {
 struct DeviceTables {
 	std::vector<uint8_t> pid_of;
 	std::vector<uint2> rows;
+	std::vector<uint32_t> pid_vox;
 	std::vector<uint32_t> codes;
 	std::vector<hcm_custom_mesh> meshes;
 	std::vector<uint8_t> tex_flags;
@@ -52,55 +53,66 @@ int build_tables(const hcm_registry *r, DeviceTables *t) {
 		uint32_t id = tex & 0x1fffu;
 		return id < r->n_tex ? r->tex_flags[id] : 0u;
 	};
+	// face class of a texture for BlockTexture::Show: empty, opaque, transparent, liquid
+	auto tex_class = [&](uint16_t tex) -> uint32_t {
+		if ((tex & 0x1fffu) == 0)
+			return kClassEmpty;
+		uint32_t f = tex_flags(tex);
+		if (f & HCM_TEX_LIQUID)
+			return kClassLiquid;
+		return (f & HCM_TEX_TRANSPARENT) ? kClassTransparent : kClassOpaque;
+	};
 	// texture codes: code 0 is the empty texture value 0; codes are unique per full texture value
 	std::vector<uint16_t> code_tex{0};
-	auto code_of = [&](uint16_t tex) -> int {
+	auto code_of = [&](uint16_t tex) -> uint32_t {
 		for (size_t i = 0; i < code_tex.size(); ++i)
 			if (code_tex[i] == tex)
-				return (int)i;
+				return (uint32_t)i;
 		code_tex.push_back(tex);
-		return (int)code_tex.size() - 1;
+		return (uint32_t)code_tex.size() - 1;
+	};
+	auto add_row = [&](const uint16_t tex[6], uint32_t dp, uint32_t ip, uint32_t mesh, uint32_t flags) {
+		uint32_t tc[6], cls = 0;
+		for (int f = 0; f < 6; ++f) {
+			tc[f] = code_of(tex[f]);
+			cls |= tex_class(tex[f]) << (2 * f);
+		}
+		const uint32_t pid = (uint32_t)t->rows.size();
+		t->rows.push_back(make_uint2(tc[0] | tc[1] << 8 | tc[2] << 16 | tc[3] << 24, tc[4] | tc[5] << 8 | mesh << 24));
+		t->pid_vox.push_back((dp ? kVoxDirect : 0u) | (ip ? kVoxIndirect : 0u) | cls << kVoxClassShift | flags |
+		                     pid << kVoxPidShift);
 	};
 	t->rows.clear();
+	t->pid_vox.clear();
 	for (uint32_t i = 0; i < r->n_rows; ++i) {
 		const hcm_block_row &row = r->rows[i];
-		int tc[6];
-		for (int f = 0; f < 6; ++f)
-			tc[f] = code_of(row.tex[f]);
-		uint32_t flags = 0;
-		if (row.transparency == 2 || row.collision != 1)
-			flags |= kRowDirectPass;
-		if (row.transparency != 0)
-			flags |= kRowIndirectPass;
-		uint32_t mesh = 0xffu;
+		uint32_t mesh = 0xffu, flags = 0;
 		if (row.mesh != HCM_NO_MESH) {
 			if (row.mesh >= r->n_meshes)
 				return HCM_ERR_INVALID;
 			mesh = row.mesh;
-			flags |= kRowHasMesh;
+			flags |= kVoxMesh;
 			if (r->meshes[row.mesh].kind == HCM_MESH_LIQUID_FLOW)
-				flags |= kRowLiquidFlow;
+				flags |= kVoxFlow;
 			else if (r->meshes[row.mesh].face_count > HCM_MAX_FACES)
 				return HCM_ERR_INVALID;
 		}
-		t->rows.push_back(make_uint2(uint32_t(tc[0]) | uint32_t(tc[1]) << 8 | uint32_t(tc[2]) << 16 |
-		                                 uint32_t(tc[3]) << 24,
-		                             uint32_t(tc[4]) | uint32_t(tc[5]) << 8 | flags << 16 | mesh << 24));
+		add_row(row.tex, row.transparency == 2 || row.collision != 1, row.transparency != 0, mesh, flags);
 	}
 	// dynamic liquid-flow textures: 16 = all faces liquid; else bits right, left, front, back full
+	// (the voxel keeps its own light-pass bits; only the face textures come from these rows)
 	t->flow_pid_base = r->n_rows;
-	const uint32_t fc = (uint32_t)code_of(r->liquid_flow_tex);
+	const uint16_t ft = r->liquid_flow_tex;
 	for (uint32_t p = 0; p < (uint32_t)kFlowPatterns; ++p) {
-		uint32_t tc[6];
+		uint16_t tex[6];
 		if (p == 16)
 			for (int f = 0; f < 6; ++f)
-				tc[f] = fc;
+				tex[f] = ft;
 		else {
-			tc[0] = (p & 1) ? fc : 0, tc[1] = (p & 2) ? fc : 0, tc[2] = 0, tc[3] = fc;
-			tc[4] = (p & 4) ? fc : 0, tc[5] = (p & 8) ? fc : 0;
+			tex[0] = (p & 1) ? ft : 0, tex[1] = (p & 2) ? ft : 0, tex[2] = 0, tex[3] = ft;
+			tex[4] = (p & 4) ? ft : 0, tex[5] = (p & 8) ? ft : 0;
 		}
-		t->rows.push_back(make_uint2(tc[0] | tc[1] << 8 | tc[2] << 16 | tc[3] << 24,
-		                             tc[4] | tc[5] << 8 | (kRowDirectPass | kRowIndirectPass) << 16 | 0xffu << 24));
+		add_row(tex, 0, 0, 0xffu, 0);
 	}
 	if (code_tex.size() > (size_t)kMaxCodes)
 		return HCM_ERR_INVALID;
@@ -145,6 +157,7 @@ struct hcm_ctx {
 	// device registry buffers
 	uint8_t *d_pid_of = nullptr;
 	uint2 *d_rows = nullptr;
+	uint32_t *d_pid_vox = nullptr;
 	uint32_t *d_codes = nullptr;
 	hcm_custom_mesh *d_meshes = nullptr;
 	uint8_t *d_tex_flags = nullptr;
@@ -215,6 +228,7 @@ static void free_ctx(hcm_ctx *c) {
 		return;
 	dfree(c->d_pid_of);
 	dfree(c->d_rows);
+	dfree(c->d_pid_vox);
 	dfree(c->d_codes);
 	dfree(c->d_meshes);
 	dfree(c->d_tex_flags);
@@ -252,11 +266,13 @@ static int create_impl(hcm_ctx *c, const hcm_registry *registry, uint64_t pool_q
 
 	HCM_HIP(dmalloc(&c->d_pid_of, t.pid_of.size()));
 	HCM_HIP(dmalloc(&c->d_rows, t.rows.size()));
+	HCM_HIP(dmalloc(&c->d_pid_vox, t.pid_vox.size()));
 	HCM_HIP(dmalloc(&c->d_codes, t.codes.size()));
 	HCM_HIP(dmalloc(&c->d_meshes, t.meshes.size()));
 	HCM_HIP(dmalloc(&c->d_tex_flags, t.tex_flags.size()));
 	HCM_HIP(hipMemcpy(c->d_pid_of, t.pid_of.data(), t.pid_of.size(), hipMemcpyHostToDevice));
 	HCM_HIP(hipMemcpy(c->d_rows, t.rows.data(), t.rows.size() * sizeof(uint2), hipMemcpyHostToDevice));
+	HCM_HIP(hipMemcpy(c->d_pid_vox, t.pid_vox.data(), t.pid_vox.size() * 4, hipMemcpyHostToDevice));
 	HCM_HIP(hipMemcpy(c->d_codes, t.codes.data(), t.codes.size() * 4, hipMemcpyHostToDevice));
 	if (!t.meshes.empty())
 		HCM_HIP(hipMemcpy(c->d_meshes, t.meshes.data(), t.meshes.size() * sizeof(hcm_custom_mesh),
@@ -265,6 +281,7 @@ static int create_impl(hcm_ctx *c, const hcm_registry *registry, uint64_t pool_q
 		HCM_HIP(hipMemcpy(c->d_tex_flags, t.tex_flags.data(), t.tex_flags.size(), hipMemcpyHostToDevice));
 	c->reg.pid_of = c->d_pid_of;
 	c->reg.rows = c->d_rows;
+	c->reg.pid_vox = c->d_pid_vox;
 	c->reg.codes = c->d_codes;
 	c->reg.meshes = c->d_meshes;
 	c->reg.tex_flags = c->d_tex_flags;
@@ -357,6 +374,17 @@ static int persistent_grid(hcm_ctx *c, uint32_t threads) {
 		per_cu = std::min<uint32_t>(per_cu, 2048 / threads);
 	}
 	return (int)(per_cu * (uint32_t)c->n_cu);
+}
+
+int hcm::ctx_info(hcm_ctx *ctx, uint32_t threads, CtxInfo *out) {
+	if (!ctx || !out)
+		return HCM_ERR_INVALID;
+	std::lock_guard<std::mutex> lk(ctx->mu);
+	out->reg = ctx->reg;
+	out->n_cu = ctx->n_cu;
+	out->threads = threads ? threads : ctx->tuning.threads;
+	out->grid = (uint32_t)persistent_grid(ctx, out->threads);
+	return HCM_OK;
 }
 
 // Assemble one job's meshes from a device pool into a caller's host buffers.

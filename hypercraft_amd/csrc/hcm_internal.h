@@ -12,11 +12,15 @@ constexpr int kMaxPid = 128;   // registry rows + dynamic liquid-texture rows ke
 constexpr int kMaxCodes = 128; // distinct face textures kept in LDS
 constexpr int kFlowPatterns = 17;
 
-// Row word y layout: tc4 | tc5 << 8 | flags << 16 | mesh << 24 (mesh 0xFF = none)
-constexpr uint32_t kRowDirectPass = 1u;   // VerticalLightPass || collision != solid (BlockMeshAlgo.hpp:119)
-constexpr uint32_t kRowIndirectPass = 2u; // transparency != opaque (Block.hpp:70)
-constexpr uint32_t kRowHasMesh = 4u;
-constexpr uint32_t kRowLiquidFlow = 8u;
+// Voxel word staged in LDS per halo voxel: the light byte, then the per-row template below.
+constexpr uint32_t kVoxDirect = 1u << 8;   // VerticalLightPass || collision != solid (BlockMeshAlgo.hpp:119)
+constexpr uint32_t kVoxIndirect = 1u << 9; // transparency != opaque (block/include/block/Block.hpp:70)
+constexpr int kVoxClassShift = 10;         // 2-bit face class per BlockFace f at bit 10 + 2 f
+constexpr uint32_t kVoxMesh = 1u << 22;    // has a custom mesh
+constexpr uint32_t kVoxFlow = 1u << 23;    // liquid flow (dynamic mesh and face textures)
+constexpr int kVoxPidShift = 24;           // texture row
+// Face classes for BlockTexture::Show (texture/include/texture/BlockTexture.hpp:68-76)
+constexpr uint32_t kClassEmpty = 0, kClassOpaque = 1, kClassTransparent = 2, kClassLiquid = 3;
 
 // Texture-code flags (code word = texture value | flags << 16)
 constexpr uint32_t kCodeTransparent = 1u;
@@ -25,8 +29,9 @@ constexpr uint32_t kCodeLiquid = 4u;
 constexpr uint32_t kCodeEmpty = 8u;
 
 struct DevReg {
-	const uint8_t *pid_of;         // [65536] Block::m_data -> pid
-	const uint2 *rows;             // [n_pid]: x = tc0..tc3, y = tc4 | tc5 << 8 | flags << 16 | mesh << 24
+	const uint8_t *pid_of;         // [65536] Block::m_data -> texture row (pid)
+	const uint2 *rows;             // [n_pid]: x = tc0..tc3 (8 bits each), y = tc4 | tc5 << 8 | mesh << 24
+	const uint32_t *pid_vox;       // [n_pid]: voxel-word template (bits 8..31 above)
 	const uint32_t *codes;         // [n_codes]
 	const hcm_custom_mesh *meshes; // [n_meshes]
 	const uint8_t *tex_flags;      // [n_tex] by texture id
@@ -59,9 +64,24 @@ struct MeshParams {
 	hcm_job_result *results;
 };
 
+// One voxel edit of a device chunk store: slot, chunk-local index, block, light.
+struct VoxelEdit {
+	uint32_t slot, local, block, light;
+};
+
 // Launchers (hcm_kernels.hip)
 hipError_t launch_reset(Counters *ctr, hipStream_t stream);
 hipError_t launch_mesh(const MeshParams &p, int threads, int grid, hipStream_t stream);
+hipError_t launch_edits(uint16_t *blocks, uint8_t *light, const VoxelEdit *edits, uint32_t n, hipStream_t stream);
 int mesh_lds_bytes(int threads);
+
+// Context internals the serving loop needs (hcm_api.cpp)
+struct CtxInfo {
+	DevReg reg;
+	int n_cu;
+	uint32_t threads;
+	uint32_t grid; // persistent grid for `threads`
+};
+int ctx_info(hcm_ctx *ctx, uint32_t threads, CtxInfo *out);
 
 } // namespace hcm

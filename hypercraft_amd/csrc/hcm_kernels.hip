@@ -1,22 +1,26 @@
 // hcm_kernels.hip -- gfx950 chunk-meshing kernels.
 //
-// One workgroup meshes one 16^3 chunk at a time (a persistent workgroup takes chunk jobs from a work
-// queue until it is empty). Per chunk:
-//   1. stage the 18^3 block/light halo from the 27 chunk slots into LDS as one word per voxel
-//      (light | direct-pass << 8 | indirect-pass << 9 | texture row << 16), plus bitmaps of cells with
-//      custom meshes and of liquid-flow cells; a chunk whose halo is one block value exits here;
-//   2. liquid-flow cells get their per-cell face textures (the reference's m_dynamic_block_textures);
-//   3. faces: one wavefront per (axis, slice, direction) task computes, 64 cells at a time, the face
-//      visibility (BlockTexture::Show), the per-vertex AO / smooth light word (light4_init) and, by
-//      ballot, the 16x16 row masks the greedy merge needs: visible, equal-to-right, equal-to-below,
-//      transparent-pass;
-//   4. one lane per task replays the reference's greedy merge on those bit masks (u-major, v-minor,
-//      width first, then height) and counts quads per pass; custom-mesh faces are counted per cell;
-//   5. block-wide prefix sums give every quad its index in its pass and its position in the
-//      reference's global emission order; one atomic per chunk allocates the output range;
-//   6. quads are described into an LDS list (in windows) and written densely, one lane per quad,
-//      as packed BlockVertex words and u16 indices, with the reference's 65,532-vertex mesh split,
-//      per-mesh AABBs and mesh order.
+// One workgroup meshes one 16^3 chunk at a time; a persistent workgroup takes chunk jobs from a work
+// queue until it is empty. Per chunk:
+//   1. staging: one thread per 18-voxel x-row of the 18^3 halo loads the row from its chunk slots with
+//      vector loads and writes one LDS word per voxel: light byte, direct/indirect light-pass bits,
+//      a 2-bit face class per face (empty / opaque / transparent / liquid) and the texture row;
+//      bitmaps mark cells with custom meshes and liquid-flow cells. A chunk whose halo holds a single
+//      block value and no custom mesh has no faces and stops here.
+//   2. liquid-flow cells take their per-cell face textures (the reference's m_dynamic_block_textures).
+//   3. visibility: for every (axis, slice, direction) task a wavefront classifies 64 cells per round
+//      (BlockTexture::Show on the class bits; texture codes only for transparent against transparent)
+//      and writes the 16x16 visible mask by ballot.
+//   4. light words: the visible faces are numbered densely (task-major, row-major, as the reference
+//      scans them), light4_init is evaluated once per visible face, and the equal-to-right,
+//      equal-to-below and transparent-pass masks are set from neighbouring faces' words.
+//   5. greedy merge: one lane per task replays the reference's merge row by row on the bit masks:
+//      the quads starting in a row are the row's uncovered visible runs split where the face word
+//      changes; each extends downward while every column of it continues (equal and uncovered).
+//   6. block-wide prefix sums give every quad its index in its pass and its position in the
+//      reference's emission order; one atomic per chunk allocates its output range; quads are then
+//      described into an LDS list and written one lane per quad as packed BlockVertex words and u16
+//      indices, with the reference's 65,532-vertex mesh split, per-mesh AABBs and mesh order.
 // All arithmetic is integer. References are to client/include/client/BlockMeshAlgo.hpp unless noted.
 #include <hip/hip_runtime.h>
 
@@ -26,8 +30,10 @@ namespace hcm {
 
 constexpr int kHalo = 18;
 constexpr int kHaloVox = kHalo * kHalo * kHalo;
+constexpr int kHaloRows = kHalo * kHalo;
 constexpr int kTasks = 102; // 3 axes x 17 slices x 2 directions
-constexpr int kListCap = 1024;
+constexpr int kFaceCap = 1024;
+constexpr int kListCap = 1536;
 constexpr uint32_t kSplitQuads = 16383u; // a BlockMesh holds at most 65,532 vertices (:273-279, :397-403)
 constexpr int kMaxMeshPerPass = 8;
 
@@ -39,10 +45,11 @@ __host__ __device__ constexpr int axis_stride(int a) { return a == 0 ? kSX : (a 
 __host__ __device__ constexpr int u_axis(int a) { return a == 1 ? 2 : 1; }
 __host__ __device__ constexpr int v_axis(int a) { return a == 0 ? 2 : 0; }
 
-// The light4_init cell pattern (kLookup3v, :75-108). Vertex k of face F reads, in the plane of the
-// front cell, a side cell c0 offset along plane axis Q, the corner cell c1 offset along P and Q, and a
-// side cell c2 offset along P; the signs of (P, Q) per vertex follow one of two windings. The host-side
-// static_asserts below check this against the reference's literal table.
+// The light4_init cell pattern (kLookup3v, :75-108). Vertex k of face F reads, in the plane of the front
+// cell, a side cell c0 offset along plane axis Q, the corner cell c1 offset along P and Q, and a side cell
+// c2 offset along P; (P, Q) = (y, z) for x faces, (x, z) for y faces, (x, y) for z faces. The signs along
+// (P, Q) are (-,-), (+,-), (+,+), (-,+) for faces 0, 3, 4 ("winding a") and (-,-), (-,+), (+,+), (+,-) for
+// faces 1, 2, 5. The static_assert below checks this against the reference's literal table.
 __host__ __device__ constexpr int plane_p(int F) { return (F >> 1) == 0 ? 1 : 0; }
 __host__ __device__ constexpr int plane_q(int F) { return (F >> 1) == 2 ? 1 : 2; }
 __host__ __device__ constexpr bool winding_a(int F) { return F == 0 || F == 3 || F == 4; }
@@ -53,14 +60,9 @@ __host__ __device__ constexpr int corner_sq(int F, int k) {
 	return k == 0 ? -1 : (k == 2 ? 1 : (k == 1 ? (winding_a(F) ? -1 : 1) : (winding_a(F) ? 1 : -1)));
 }
 __host__ __device__ constexpr int normal_sign(int F) { return (F & 1) ? -1 : 1; }
-// Offset of (axis components) d[3] in the 3 axes for cell b of vertex k of face F, as (dx, dy, dz).
 __host__ __device__ constexpr int corner_comp(int F, int k, int b, int axis) {
-	return (axis == (F >> 1) ? normal_sign(F) : 0) +
-	       (axis == plane_p(F) && b >= 1 ? corner_sp(F, k) : 0) +
+	return (axis == (F >> 1) ? normal_sign(F) : 0) + (axis == plane_p(F) && b >= 1 ? corner_sp(F, k) : 0) +
 	       (axis == plane_q(F) && b <= 1 ? corner_sq(F, k) : 0);
-}
-__host__ __device__ constexpr int corner_off(int F, int k, int b) {
-	return corner_comp(F, k, b, 0) * kSX + corner_comp(F, k, b, 1) * kSY + corner_comp(F, k, b, 2) * kSZ;
 }
 
 namespace check {
@@ -100,12 +102,22 @@ __device__ __forceinline__ uint32_t local16(int x, int y, int z) {
 }
 __device__ __forceinline__ int hidx(int x, int y, int z) { return (x + 1) * kSX + (y + 1) * kSY + (z + 1) * kSZ; }
 
+struct FaceStage { // dense visible faces of a group of tasks
+	uint32_t desc[kFaceCap]; // task | u << 7 | v << 11
+	uint2 key[kFaceCap];     // Light4 words; key.y top byte = texture code
+};
+
 template <int NT> struct Smem {
 	uint32_t vox[kHaloVox];
+	// per task and mask row: visible, equal-to-right, equal-to-below, transparent pass
 	uint16_t ne[kTasks][16], eh[kTasks][16], ev[kTasks][16], tp[kTasks][16];
-	uint16_t cov[kTasks][16];
-	uint2 list[kListCap];
+	union {
+		FaceStage fs;
+		uint2 list[kListCap];
+	};
+	uint32_t tpre[kTasks + 2]; // prefix of visible faces per task
 	uint2 rows[kMaxPid];
+	uint32_t pid_vox[kMaxPid];
 	uint32_t codes[kMaxCodes];
 	uint32_t custom_bits[128]; // in-chunk cells with a custom mesh
 	uint32_t flow_bits[(kHaloVox + 31) / 32];
@@ -114,7 +126,7 @@ template <int NT> struct Smem {
 	uint64_t scan[NT / 64 + 1];
 	uint32_t slots[27];
 	uint32_t job, flags, n_events, status;
-	uint32_t n0, n1, cust0, cust1, pool_base, mesh_base, n_mesh0, n_mesh1;
+	uint32_t pool_base, mesh_base, n_mesh0, n_mesh1, group_end;
 };
 
 constexpr uint32_t kFlagNonUniform = 1u, kFlagCustom = 2u, kFlagFlow = 4u;
@@ -123,80 +135,68 @@ constexpr uint32_t kFlagNonUniform = 1u, kFlagCustom = 2u, kFlagFlow = 4u;
 // small helpers
 // ------------------------------------------------------------------------------------------------
 template <int NT> __device__ __forceinline__ uint32_t row_code(const Smem<NT> &s, uint32_t pid, int face) {
-	uint2 r = s.rows[pid];
-	uint32_t w = face < 4 ? r.x : r.y;
+	const uint2 r = s.rows[pid];
+	const uint32_t w = face < 4 ? r.x : r.y;
 	return (w >> (8 * (face & 3))) & 0xffu;
 }
 
-// BlockTexture::Show (texture/include/texture/BlockTexture.hpp:68-76) on texture codes.
-template <int NT> __device__ __forceinline__ bool show(const Smem<NT> &s, uint32_t a, uint32_t b) {
-	uint32_t fa = s.codes[a] >> 16, fb = s.codes[b] >> 16;
-	if ((fa & kCodeEmpty) || a == b)
-		return false;
-	bool ta = fa & kCodeTransparent, tb = fb & kCodeTransparent;
-	if (!ta && !tb)
-		return false;
-	if ((fa & kCodeLiquid) && !(fb & kCodeEmpty))
-		return false;
-	return !ta || tb || (fb & kCodeLiquid);
+__device__ __forceinline__ uint32_t face_class(uint32_t vox, int face) {
+	return (vox >> (kVoxClassShift + 2 * face)) & 3u;
 }
 
 // (s << 2) / cnt for cnt in 1..4 and s <= 60, exactly: multiply by ceil(65536 / cnt) and shift.
 __device__ __forceinline__ uint32_t div_cnt(uint32_t x, uint32_t cnt) {
-	uint32_t r = cnt == 1 ? 65536u : (cnt == 2 ? 32768u : (cnt == 3 ? 21846u : 16384u));
+	const uint32_t r = cnt == 1 ? 65536u : (cnt == 2 ? 32768u : (cnt == 3 ? 21846u : 16384u));
 	return (x * r) >> 16;
 }
 
-// light4_init (:48-163) for face F of the block at halo index self. Returns the packed Light4:
+// One vertex of light4_init (:114-162) from its side, corner and side cells and the front cell.
+__device__ __forceinline__ void light_vertex(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t pass3, uint32_t fs,
+                                             uint32_t ft, uint32_t &ao, uint32_t &sun, uint32_t &torch) {
+	const uint32_t d0 = (c0 >> 8) & 1u, d1 = (c1 >> 8) & 1u, d2 = (c2 >> 8) & 1u;
+	const uint32_t i0 = (c0 >> 9) & 1u, i1 = (c1 >> 9) & 1u, i2 = (c2 >> 9) & 1u;
+	if ((!d0 && !d2) || (!d1 && !pass3))
+		ao = 0;
+	else {
+		const int a = 3 - int(!d0) - int(!d1) - int(!d2) - int(!pass3);
+		ao = a < 0 ? 0u : uint32_t(a);
+	}
+	uint32_t cnt = 1, ss = fs, tt = ft;
+	if (i0 | i2) {
+		cnt += i0 + i1 + i2;
+		ss += (i0 ? (c0 & 15u) : 0u) + (i1 ? (c1 & 15u) : 0u) + (i2 ? (c2 & 15u) : 0u);
+		tt += (i0 ? ((c0 >> 4) & 15u) : 0u) + (i1 ? ((c1 >> 4) & 15u) : 0u) + (i2 ? ((c2 >> 4) & 15u) : 0u);
+	}
+	sun = div_cnt(ss << 2, cnt);
+	torch = div_cnt(tt << 2, cnt);
+}
+
+// light4_init (:48-163) for face `face` of the block at halo index self. Returns the packed Light4:
 //   x = sun0 | sun1 << 6 | sun2 << 12 | sun3 << 18 | ao4 << 24,   y = torch0 | ... | torch3 << 18.
-template <int F> __device__ __forceinline__ uint2 light4(const uint32_t *vox, int self) {
-	constexpr int A = F >> 1;
-	const uint32_t vf = vox[self + normal_sign(F) * axis_stride(A)];
-	const uint32_t pass3 = (vf >> 8) & 1u;
-	const uint32_t fs = vf & 15u, ft = (vf >> 4) & 15u;
+__device__ __forceinline__ uint2 light4(const uint32_t *vox, int face, int self) {
+	const int a = face >> 1;
+	const int sn = a == 0 ? kSX : (a == 1 ? kSY : kSZ);
+	const int sp = a == 0 ? kSY : kSX; // plane axis P
+	const int sq = a == 2 ? kSY : kSZ; // plane axis Q
+	const int f = self + ((face & 1) ? -sn : sn);
+	const uint32_t F = vox[f];
+	const uint32_t qm = vox[f - sq], qp = vox[f + sq], pm = vox[f - sp], pp = vox[f + sp];
+	const uint32_t mm = vox[f - sp - sq], mp = vox[f - sp + sq], pmq = vox[f + sp - sq], ppq = vox[f + sp + sq];
+	const bool wa = face == 0 || face == 3 || face == 4;
+	const uint32_t pass3 = (F >> 8) & 1u, fs = F & 15u, ft = (F >> 4) & 15u;
+	uint32_t ao[4], sun[4], torch[4];
+	light_vertex(qm, mm, pm, pass3, fs, ft, ao[0], sun[0], torch[0]);  // (P, Q) signs (-,-)
+	light_vertex(qp, ppq, pp, pass3, fs, ft, ao[2], sun[2], torch[2]); // (+,+)
+	light_vertex(wa ? qm : qp, wa ? pmq : mp, wa ? pp : pm, pass3, fs, ft, ao[1], sun[1], torch[1]);
+	light_vertex(wa ? qp : qm, wa ? mp : pmq, wa ? pm : pp, pass3, fs, ft, ao[3], sun[3], torch[3]);
 	uint32_t lo = 0, hi = 0, ao4 = 0;
 #pragma unroll
 	for (int k = 0; k < 4; ++k) {
-		const uint32_t c0 = vox[self + corner_off(F, k, 0)];
-		const uint32_t c1 = vox[self + corner_off(F, k, 1)];
-		const uint32_t c2 = vox[self + corner_off(F, k, 2)];
-		const uint32_t d0 = (c0 >> 8) & 1u, d1 = (c1 >> 8) & 1u, d2 = (c2 >> 8) & 1u;
-		const uint32_t i0 = (c0 >> 9) & 1u, i1 = (c1 >> 9) & 1u, i2 = (c2 >> 9) & 1u;
-		uint32_t ao;
-		if ((!d0 && !d2) || (!d1 && !pass3))
-			ao = 0;
-		else {
-			int a = 3 - int(!d0) - int(!d1) - int(!d2) - int(!pass3);
-			ao = a < 0 ? 0u : uint32_t(a);
-		}
-		uint32_t cnt = 1, ss = fs, tt = ft;
-		if (i0 | i2) {
-			cnt += i0 + i1 + i2;
-			ss += (i0 ? (c0 & 15u) : 0u) + (i1 ? (c1 & 15u) : 0u) + (i2 ? (c2 & 15u) : 0u);
-			tt += (i0 ? ((c0 >> 4) & 15u) : 0u) + (i1 ? ((c1 >> 4) & 15u) : 0u) + (i2 ? ((c2 >> 4) & 15u) : 0u);
-		}
-		lo |= div_cnt(ss << 2, cnt) << (6 * k);
-		hi |= div_cnt(tt << 2, cnt) << (6 * k);
-		ao4 |= ao << (2 * k);
+		lo |= sun[k] << (6 * k);
+		hi |= torch[k] << (6 * k);
+		ao4 |= ao[k] << (2 * k);
 	}
 	return make_uint2(lo | (ao4 << 24), hi);
-}
-
-__device__ __forceinline__ uint2 light4_rt(const uint32_t *vox, int face, int self) {
-	switch (face) {
-	case 0:
-		return light4<0>(vox, self);
-	case 1:
-		return light4<1>(vox, self);
-	case 2:
-		return light4<2>(vox, self);
-	case 3:
-		return light4<3>(vox, self);
-	case 4:
-		return light4<4>(vox, self);
-	default:
-		return light4<5>(vox, self);
-	}
 }
 
 __device__ __forceinline__ uint32_t l4_sun(uint2 l, int k) { return (l.x >> (6 * k)) & 63u; }
@@ -205,8 +205,8 @@ __device__ __forceinline__ uint32_t l4_ao(uint2 l, int k) { return (l.x >> (24 +
 
 // Light4::GetFlip (:38-43): sunlight and AO only.
 __device__ __forceinline__ bool l4_flip(uint2 l) {
-	int d02 = int(l4_ao(l, 0) + 1) * int(l4_sun(l, 0) + 1) - int(l4_ao(l, 2) + 1) * int(l4_sun(l, 2) + 1);
-	int d13 = int(l4_ao(l, 1) + 1) * int(l4_sun(l, 1) + 1) - int(l4_ao(l, 3) + 1) * int(l4_sun(l, 3) + 1);
+	const int d02 = int(l4_ao(l, 0) + 1) * int(l4_sun(l, 0) + 1) - int(l4_ao(l, 2) + 1) * int(l4_sun(l, 2) + 1);
+	const int d13 = int(l4_ao(l, 1) + 1) * int(l4_sun(l, 1) + 1) - int(l4_ao(l, 3) + 1) * int(l4_sun(l, 3) + 1);
 	return abs(d02) < abs(d13);
 }
 
@@ -220,7 +220,7 @@ template <int NT> __device__ __forceinline__ uint64_t block_excl_scan(Smem<NT> &
 	uint64_t inc = v;
 #pragma unroll
 	for (int d = 1; d < 64; d <<= 1) {
-		uint32_t lo = __shfl_up((uint32_t)inc, d), hi = __shfl_up((uint32_t)(inc >> 32), d);
+		const uint32_t lo = __shfl_up((uint32_t)inc, d), hi = __shfl_up((uint32_t)(inc >> 32), d);
 		if (lane >= d)
 			inc += (uint64_t)lo | ((uint64_t)hi << 32);
 	}
@@ -230,14 +230,14 @@ template <int NT> __device__ __forceinline__ uint64_t block_excl_scan(Smem<NT> &
 	if (threadIdx.x == 0) {
 		uint64_t run = 0;
 		for (int w = 0; w < NT / 64; ++w) {
-			uint64_t t = s.scan[w];
+			const uint64_t t = s.scan[w];
 			s.scan[w] = run;
 			run += t;
 		}
 		s.scan[NT / 64] = run;
 	}
 	__syncthreads();
-	uint64_t r = s.scan[wave] + inc - v;
+	const uint64_t r = s.scan[wave] + inc - v;
 	*total = s.scan[NT / 64];
 	__syncthreads();
 	return r;
@@ -279,7 +279,7 @@ __device__ __forceinline__ uint32_t flow_vh(int nb, int a, int b, int c) {
 }
 
 __device__ __forceinline__ Flow flow_heights(const MeshParams &p, const uint32_t *slots, int x, int y, int z,
-                                             uint32_t nb_height) {
+                                          uint32_t nb_height) {
 	const uint32_t src = p.reg.src_id, fid = p.reg.flow_id;
 	// neighbours (:291-300): 0 (-1,-1) 1 (0,-1) 2 (1,-1) 3 (-1,0) 4 (1,0) 5 (-1,1) 6 (0,1) 7 (1,1) in (x, z)
 	const uint32_t b0 = global_block(p, slots, x - 1, y, z - 1), b1 = global_block(p, slots, x, y, z - 1);
@@ -373,29 +373,29 @@ __device__ hcm_mesh_face flow_face(const Flow &f, int kind, uint16_t tex) {
 // custom meshes (:198-306)
 // ------------------------------------------------------------------------------------------------
 struct CustomCell {
-	uint32_t count;     // faces emitted
-	uint32_t trans;     // bit j: face j goes to the transparent pass
-	uint32_t mesh;      // registry mesh index
-	uint32_t flow;      // 1 if liquid flow
+	uint32_t count; // faces emitted
+	uint32_t trans; // bit j: face j goes to the transparent pass
+	uint32_t mesh;  // registry mesh index
+	uint32_t flow;  // 1 if liquid flow
 	Flow fl;
 	uint32_t flow_set; // liquid flow: emitted face kinds
 };
 
 __device__ __forceinline__ bool tex_trans_pass(const MeshParams &p, uint16_t t) {
-	uint32_t id = t & 0x1fffu;
+	const uint32_t id = t & 0x1fffu;
 	return id < p.reg.n_tex && (p.reg.tex_flags[id] & HCM_TEX_TRANS_PASS);
 }
 
-template <int NT>
-__device__ __forceinline__ CustomCell custom_cell(const MeshParams &p, const Smem<NT> &s, int cell) {
+template <int NT> __device__ __forceinline__ CustomCell custom_cell(const MeshParams &p, const Smem<NT> &s, int cell) {
 	const int x = cell & 15, z = (cell >> 4) & 15, y = cell >> 8;
 	const uint16_t b = p.blocks[(size_t)s.slots[26] * HCM_CHUNK_VOXELS + (uint32_t)cell];
-	const uint2 row = s.rows[p.reg.pid_of[b]];
+	const uint32_t pid = p.reg.pid_of[b];
 	CustomCell c;
-	c.mesh = row.y >> 24;
-	c.flow = (row.y >> 16) & kRowLiquidFlow ? 1u : 0u;
+	c.mesh = s.rows[pid].y >> 24;
+	c.flow = (s.pid_vox[pid] & kVoxFlow) ? 1u : 0u;
 	c.trans = 0;
 	c.flow_set = 0;
+	c.fl.h = 0, c.fl.liq_side = 0;
 	const hcm_custom_mesh &m = p.reg.meshes[c.mesh];
 	if (c.flow) {
 		c.fl = flow_heights(p, s.slots, x, y, z, m.flow_height);
@@ -422,144 +422,26 @@ __device__ __forceinline__ hcm_mesh_face custom_face(const MeshParams &p, const 
 }
 
 // ------------------------------------------------------------------------------------------------
-// faces of one (axis, slice, direction) task: one wavefront, 4 rounds of 4 mask rows
-// ------------------------------------------------------------------------------------------------
-template <int F, int NT> __device__ void face_task(Smem<NT> &s, int t, int sl) {
-	constexpr int A = F >> 1, INV = F & 1;
-	constexpr int SA = axis_stride(A), SU = axis_stride(u_axis(A)), SV = axis_stride(v_axis(A));
-	const int lane = threadIdx.x & 63;
-	const int r = lane >> 4, v = lane & 15;
-	const int layer = INV ? sl : sl - 1; // the block owning the face (:330-339)
-	uint32_t prev_lo = 0, prev_hi = 0;
-	bool prev_vis = false;
-	uint64_t prev_vt = 0;
-	for (int q = 0; q < 4; ++q) {
-		const int u = 4 * q + r;
-		const int self = (layer + 1) * SA + (u + 1) * SU + (v + 1) * SV;
-		const uint32_t vs = s.vox[self], vo = s.vox[self + (INV ? -SA : SA)];
-		const uint32_t tcs = row_code(s, vs >> 16, F), tco = row_code(s, vo >> 16, F ^ 1);
-		const bool vis = show(s, tcs, tco);
-		const uint64_t vis_b = ballot64(vis);
-		uint32_t lo = 0, hi = 0;
-		uint64_t eh_b = 0, ev_b = 0, tp_b = 0;
-		if (vis_b) {
-			if (vis) {
-				uint2 l = light4<F>(s.vox, self);
-				lo = l.x;
-				hi = l.y | (tcs << 24);
-			}
-			const uint32_t rlo = __shfl_down(lo, 1, 16), rhi = __shfl_down(hi, 1, 16);
-			const uint32_t dlo = __shfl_down(lo, 16), dhi = __shfl_down(hi, 16);
-			eh_b = ballot64(vis && v < 15 && lo == rlo && hi == rhi);
-			ev_b = ballot64(vis && r < 3 && lo == dlo && hi == dhi);
-			tp_b = ballot64(vis && ((s.codes[tcs] >> 16) & kCodeTransPass));
-		}
-		if (q > 0) {
-			// complete the previous round's last row against this round's first row
-			const uint32_t ulo = __shfl(lo, v), uhi = __shfl(hi, v);
-			prev_vt |= ballot64(prev_vis && r == 3 && prev_lo == ulo && prev_hi == uhi);
-			if (lane == 0)
-				*reinterpret_cast<uint64_t *>(&s.ev[t][4 * (q - 1)]) = prev_vt;
-		}
-		if (lane == 0) {
-			*reinterpret_cast<uint64_t *>(&s.ne[t][4 * q]) = vis_b;
-			*reinterpret_cast<uint64_t *>(&s.eh[t][4 * q]) = eh_b;
-			*reinterpret_cast<uint64_t *>(&s.tp[t][4 * q]) = tp_b;
-		}
-		prev_lo = lo, prev_hi = hi, prev_vis = vis, prev_vt = ev_b;
-	}
-	if (lane == 0)
-		*reinterpret_cast<uint64_t *>(&s.ev[t][12]) = prev_vt;
-}
-
-template <int NT> __device__ void face_tasks(Smem<NT> &s) {
-	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-	for (int t = wave; t < kTasks; t += NT / 64) {
-		const int A = t / 34, rem = t % 34, sl = rem >> 1, inv = rem & 1;
-		if ((sl == 0 && !inv) || (sl == 16 && inv)) { // no block owns these faces (:330, :335)
-			if (lane < 16) {
-				s.ne[t][lane] = 0, s.eh[t][lane] = 0, s.ev[t][lane] = 0, s.tp[t][lane] = 0;
-			}
-			continue;
-		}
-		switch (2 * A + inv) {
-		case 0:
-			face_task<0, NT>(s, t, sl);
-			break;
-		case 1:
-			face_task<1, NT>(s, t, sl);
-			break;
-		case 2:
-			face_task<2, NT>(s, t, sl);
-			break;
-		case 3:
-			face_task<3, NT>(s, t, sl);
-			break;
-		case 4:
-			face_task<4, NT>(s, t, sl);
-			break;
-		default:
-			face_task<5, NT>(s, t, sl);
-			break;
-		}
-	}
-}
-
-// ------------------------------------------------------------------------------------------------
-// greedy merge of one task on its bit masks (:357-476): one lane per task
-// ------------------------------------------------------------------------------------------------
-struct Quad {
-	int u, v0, w, h;
-	bool trans;
-};
-
-template <int NT, typename Fn> __device__ __forceinline__ void greedy(Smem<NT> &s, int t, Fn &&on_quad) {
-	for (int u = 0; u < 16; ++u)
-		s.cov[t][u] = 0;
-	for (int u = 0; u < 16; ++u) {
-		const uint32_t ne = s.ne[t][u], eh = s.eh[t][u], tp = s.tp[t][u];
-		for (;;) {
-			const uint32_t c = s.cov[t][u];
-			const uint32_t avail = ne & ~c;
-			if (!avail)
-				break;
-			const int v0 = __ffs(avail) - 1;
-			// width: extend along V while the next cell equals and is not yet covered (:364-368)
-			const uint32_t run = eh & ~(c >> 1);
-			const int w = __ffs(~(run >> v0)) ; // 1 + number of consecutive ones from v0
-			const uint32_t M = ((1u << w) - 1u) << v0;
-			// height: extend along U while the whole row segment equals and is uncovered (:371-379)
-			int h = 1;
-			while (u + h < 16) {
-				if ((s.ev[t][u + h - 1] & M) != M || (s.cov[t][u + h] & M))
-					break;
-				++h;
-			}
-			for (int r2 = u; r2 < u + h; ++r2)
-				s.cov[t][r2] |= M;
-			on_quad(Quad{u, v0, w, h, ((tp >> v0) & 1u) != 0});
-		}
-	}
-}
-
-// ------------------------------------------------------------------------------------------------
 // writing one quad
 // ------------------------------------------------------------------------------------------------
 __device__ __forceinline__ void put_quad(const MeshParams &p, uint32_t slot, const uint32_t w0[4], const uint32_t w1[4],
-                                         uint32_t base, int pattern) {
+                                         uint32_t base, bool flip) {
 	uint4 *vp = reinterpret_cast<uint4 *>(p.pool_verts + 4 * (size_t)slot);
 	vp[0] = make_uint4(w0[0], w1[0], w0[1], w1[1]);
 	vp[1] = make_uint4(w0[2], w1[2], w0[3], w1[3]);
-	// index patterns: 0 = flip (0,1,2, 0,2,3) (:440-446) and custom (:295-301); 1 = no flip (1,2,3, 0,1,3)
-	uint32_t i0, i1, i2, i3, i4, i5;
-	if (pattern == 0)
-		i0 = 0, i1 = 1, i2 = 2, i3 = 0, i4 = 2, i5 = 3;
-	else
-		i0 = 1, i1 = 2, i2 = 3, i3 = 0, i4 = 1, i5 = 3;
+	// index patterns: flip (0,1,2, 0,2,3) (:440-446) -- also the custom-mesh pattern (:295-301) --
+	// and no flip (1,2,3, 0,1,3) (:453-459)
+	const uint32_t b0 = base, b1 = base + 1, b2 = base + 2, b3 = base + 3;
 	uint32_t *ip = reinterpret_cast<uint32_t *>(p.pool_idx + 6 * (size_t)slot);
-	ip[0] = ((base + i0) & 0xffffu) | (((base + i1) & 0xffffu) << 16);
-	ip[1] = ((base + i2) & 0xffffu) | (((base + i3) & 0xffffu) << 16);
-	ip[2] = ((base + i4) & 0xffffu) | (((base + i5) & 0xffffu) << 16);
+	if (flip) {
+		ip[0] = b0 | (b1 << 16);
+		ip[1] = b2 | (b0 << 16);
+		ip[2] = b2 | (b3 << 16);
+	} else {
+		ip[0] = b1 | (b2 << 16);
+		ip[1] = b3 | (b0 << 16);
+		ip[2] = b1 | (b3 << 16);
+	}
 }
 
 // BlockVertex word 1 (client/include/client/BlockVertex.hpp:13-14)
@@ -568,7 +450,8 @@ __device__ __forceinline__ uint32_t vword1(uint16_t tex, uint32_t face, uint32_t
 	       (torch << 24);
 }
 
-template <int NT> __device__ __forceinline__ void aabb_merge(Smem<NT> &s, int pass, int k, uint32_t lo[3], uint32_t hi[3]) {
+template <int NT>
+__device__ __forceinline__ void aabb_merge(Smem<NT> &s, int pass, int k, const uint32_t lo[3], const uint32_t hi[3]) {
 	uint32_t *a = s.aabb[pass][k];
 	atomicMin(&a[0], lo[0]);
 	atomicMin(&a[1], lo[1]);
@@ -581,89 +464,67 @@ template <int NT> __device__ __forceinline__ void aabb_merge(Smem<NT> &s, int pa
 // Mesh split bookkeeping for quad i of a pass: the vertex base its indices use (the reference reads the
 // 16-bit vertex count before splitting, so the first quad of a continuation mesh keeps base 65,532).
 __device__ __forceinline__ uint32_t index_base(uint32_t i) {
-	uint32_t k = i / kSplitQuads, j = i % kSplitQuads;
+	const uint32_t k = i / kSplitQuads, j = i % kSplitQuads;
 	return (j == 0 && k > 0) ? 65532u : 4u * j;
 }
 
-template <int F, int NT>
-__device__ __forceinline__ void emit_regular_f(const MeshParams &p, Smem<NT> &s, uint32_t desc, uint32_t slot, uint32_t pass,
-                               uint32_t i) {
-	constexpr int A = F >> 1, INV = F & 1, UA = u_axis(A), VA = v_axis(A);
-	// (U+1)%3 == V holds for the x and y axes (:384)
-	constexpr bool kHeightOnU = (INV != 0) != (A != 2);
+// Regular quad: task t = 34 * axis + 2 * slice + direction, origin (u, v0) on the mask, width w along V,
+// height h along U (:357-460).
+template <int NT>
+__device__ __forceinline__ void emit_regular(const MeshParams &p, Smem<NT> &s, uint32_t desc, uint32_t slot,
+                                          uint32_t pass, uint32_t i) {
 	const int t = desc & 127, u = (desc >> 7) & 15, v0 = (desc >> 11) & 15;
 	const int w = int((desc >> 15) & 15) + 1, h = int((desc >> 19) & 15) + 1;
-	const int sl = (t % 34) >> 1, layer = INV ? sl : sl - 1;
-	// block owning the face, and the quad origin on the slice plane (:382)
-	const int self = (layer + 1) * axis_stride(A) + (u + 1) * axis_stride(UA) + (v0 + 1) * axis_stride(VA);
-	const uint2 l = light4<F>(s.vox, self);
-	const uint32_t tc = row_code(s, s.vox[self] >> 16, F);
+	const int A = t / 34, sl = (t % 34) >> 1, inv = t & 1, face = 2 * A + inv;
+	const int ua = u_axis(A), va = v_axis(A);
+	const int layer = inv ? sl : sl - 1;
+	const int self = (layer + 1) * axis_stride(A) + (u + 1) * axis_stride(ua) + (v0 + 1) * axis_stride(va);
+	const uint2 l = light4(s.vox, face, self);
+	const uint32_t tc = row_code(s, s.vox[self] >> kVoxPidShift, face);
 	const uint16_t tex = uint16_t(s.codes[tc] & 0xffffu);
-	auto comp = [&](int axis, int ku, int kv) -> uint32_t {
-		// origin + ku * u-vector + kv * v-vector along one axis (:383-390)
-		int c = axis == A ? sl : (axis == UA ? u : v0);
-		if (kHeightOnU) {
-			c += (axis == UA ? ku * h : 0) + (axis == VA ? kv * w : 0);
-		} else {
-			c += (axis == VA ? ku * w : 0) + (axis == UA ? kv * h : 0);
-		}
-		return uint32_t(c);
-	};
+	// quad vectors (:383-390): (U+1)%3 == V holds for the x and y axes
+	const bool height_on_u = (inv != 0) != (A != 2);
+	int du[3], dv[3], o[3];
+#pragma unroll
+	for (int a = 0; a < 3; ++a) {
+		const int on_u = a == ua, on_v = a == va;
+		o[a] = a == A ? sl : (on_u ? u : v0);
+		du[a] = height_on_u ? on_u * h : on_v * w;
+		dv[a] = height_on_u ? on_v * w : on_u * h;
+	}
 	uint32_t w0[4], w1[4];
 #pragma unroll
 	for (int k = 0; k < 4; ++k) {
 		const int ku = (k == 1 || k == 2), kv = (k == 2 || k == 3);
-		w0[k] = (comp(0, ku, kv) << 4) | ((comp(1, ku, kv) << 4) << 10) | ((comp(2, ku, kv) << 4) << 20) |
-		        (uint32_t(A) << 30);
-		w1[k] = vword1(tex, uint32_t(F), l4_ao(l, k), l4_sun(l, k), l4_torch(l, k));
+		const uint32_t x = uint32_t(o[0] + ku * du[0] + kv * dv[0]) << 4;
+		const uint32_t y = uint32_t(o[1] + ku * du[1] + kv * dv[1]) << 4;
+		const uint32_t z = uint32_t(o[2] + ku * du[2] + kv * dv[2]) << 4;
+		w0[k] = x | (y << 10) | (z << 20) | (uint32_t(A) << 30);
+		w1[k] = vword1(tex, uint32_t(face), l4_ao(l, k), l4_sun(l, k), l4_torch(l, k));
 	}
-	put_quad(p, slot, w0, w1, index_base(i), l4_flip(l) ? 0 : 1);
+	put_quad(p, slot, w0, w1, index_base(i), l4_flip(l));
 	uint32_t lo[3], hi[3];
 #pragma unroll
 	for (int a = 0; a < 3; ++a) {
-		lo[a] = comp(a, 0, 0) << 4;
-		hi[a] = comp(a, 1, 1) << 4;
+		lo[a] = uint32_t(o[a]) << 4;
+		hi[a] = uint32_t(o[a] + du[a] + dv[a]) << 4;
 	}
 	aabb_merge(s, pass, i / kSplitQuads, lo, hi);
-}
-
-template <int NT>
-__device__ __forceinline__ void emit_regular(const MeshParams &p, Smem<NT> &s, uint32_t desc, uint32_t slot, uint32_t pass, uint32_t i) {
-	const int t = desc & 127, face = 2 * (t / 34) + (t & 1);
-	switch (face) {
-	case 0:
-		emit_regular_f<0>(p, s, desc, slot, pass, i);
-		break;
-	case 1:
-		emit_regular_f<1>(p, s, desc, slot, pass, i);
-		break;
-	case 2:
-		emit_regular_f<2>(p, s, desc, slot, pass, i);
-		break;
-	case 3:
-		emit_regular_f<3>(p, s, desc, slot, pass, i);
-		break;
-	case 4:
-		emit_regular_f<4>(p, s, desc, slot, pass, i);
-		break;
-	default:
-		emit_regular_f<5>(p, s, desc, slot, pass, i);
-		break;
-	}
 }
 
 __device__ __forceinline__ uint32_t lerp16(uint32_t a, uint32_t b, uint32_t t) { return a * (16u - t) + b * t; }
 __device__ __forceinline__ uint32_t interp(uint32_t l0, uint32_t l1, uint32_t l2, uint32_t l3, uint32_t h0, uint32_t h1,
                                            uint32_t h2, uint32_t h3, uint32_t du, uint32_t dv, uint32_t dw) {
 	// light4_interpolate (:165-196): trilinear in 1/16 steps, rounded up
-	uint32_t lo = lerp16(lerp16(l0, l1, du), lerp16(l3, l2, du), dv);
-	uint32_t hi = lerp16(lerp16(h0, h1, du), lerp16(h3, h2, du), dv);
-	uint32_t sum = lerp16(lo, hi, dw);
+	const uint32_t lo = lerp16(lerp16(l0, l1, du), lerp16(l3, l2, du), dv);
+	const uint32_t hi = lerp16(lerp16(h0, h1, du), lerp16(h3, h2, du), dv);
+	const uint32_t sum = lerp16(lo, hi, dw);
 	return sum / 4096u + (sum % 4096u ? 1u : 0u);
 }
 
 template <int NT>
-__device__ __forceinline__ void emit_custom(const MeshParams &p, Smem<NT> &s, uint32_t desc, uint32_t slot, uint32_t pass, uint32_t i) {
+__device__ __forceinline__ void emit_custom(const MeshParams &p, Smem<NT> &s, uint32_t desc, uint32_t slot, uint32_t pass,
+                                         uint32_t i) {
 	const int cell = desc & 4095, j = (desc >> 12) & 15;
 	const int x = cell & 15, z = (cell >> 4) & 15, y = cell >> 8;
 	const CustomCell c = custom_cell(p, s, cell);
@@ -671,11 +532,11 @@ __device__ __forceinline__ void emit_custom(const MeshParams &p, Smem<NT> &s, ui
 	const int lf = f.light_face, la = lf >> 1, sg = (lf & 1) ? -1 : 1;
 	const int pos = hidx(x, y, z);
 	// low: the face of the cell behind; high: this cell's own face if the next cell passes light (:259-267)
-	const uint2 lo4 = light4_rt(s.vox, lf, pos - sg * axis_stride(la));
-	const uint2 hi4 = ((s.vox[pos + sg * axis_stride(la)] >> 9) & 1u) ? light4_rt(s.vox, lf, pos) : lo4;
+	const uint2 lo4 = light4(s.vox, lf, pos - sg * axis_stride(la));
+	const uint2 hi4 = (s.vox[pos + sg * axis_stride(la)] & kVoxIndirect) ? light4(s.vox, lf, pos) : lo4;
 	int ua = (la + 1) % 3, va = (la + 2) % 3;
 	if (lf & 1) {
-		int tmp = ua;
+		const int tmp = ua;
 		ua = va, va = tmp;
 	}
 	const uint32_t bx = uint32_t(x) << 4, by = uint32_t(y) << 4, bz = uint32_t(z) << 4;
@@ -703,15 +564,194 @@ __device__ __forceinline__ void emit_custom(const MeshParams &p, Smem<NT> &s, ui
 		w0[k] = px | (py << 10) | (pz << 20) | (uint32_t(f.axis) << 30);
 		w1[k] = vword1(f.texture, f.render_face, ao, sun, torch);
 	}
-	put_quad(p, slot, w0, w1, index_base(i), 0);
+	put_quad(p, slot, w0, w1, index_base(i), true);
 	aabb_merge(s, pass, i / kSplitQuads, lo, hi);
+}
+
+// ------------------------------------------------------------------------------------------------
+// visibility of one (axis, slice, direction) task: one wavefront, 4 rounds of 4 mask rows
+// ------------------------------------------------------------------------------------------------
+template <int NT> __device__ __forceinline__ uint32_t task_visibility(Smem<NT> &s, int t) {
+	const int lane = threadIdx.x & 63;
+	const int A = t / 34, rem = t % 34, sl = rem >> 1, inv = rem & 1, face = 2 * A + inv;
+	uint32_t n = 0;
+	if ((sl == 0 && !inv) || (sl == 16 && inv)) { // no block owns these faces (:330, :335)
+		if (lane < 16)
+			s.ne[t][lane] = 0;
+		return 0;
+	}
+	const int sa = axis_stride(A), su = axis_stride(u_axis(A)), sv = axis_stride(v_axis(A));
+	const int layer = inv ? sl : sl - 1; // the block owning the face
+	const int r = lane >> 4, v = lane & 15;
+	const int other_off = inv ? -sa : sa;
+#pragma unroll
+	for (int q = 0; q < 4; ++q) {
+		const int self = (layer + 1) * sa + (4 * q + r + 1) * su + (v + 1) * sv;
+		const uint32_t vs = s.vox[self], vo = s.vox[self + other_off];
+		const uint32_t ca = face_class(vs, face), cb = face_class(vo, face ^ 1);
+		// BlockTexture::Show (texture/include/texture/BlockTexture.hpp:68-76) on face classes
+		bool vis = (ca == kClassOpaque && cb != kClassOpaque) || (ca == kClassLiquid && cb == kClassEmpty);
+		if (ca == kClassTransparent && cb != kClassOpaque)
+			vis = cb != kClassTransparent ||
+			      row_code(s, vs >> kVoxPidShift, face) != row_code(s, vo >> kVoxPidShift, face ^ 1);
+		const uint64_t b = ballot64(vis);
+		if (lane == 0)
+			*reinterpret_cast<uint64_t *>(&s.ne[t][4 * q]) = b;
+		n += __popcll(b);
+	}
+	return n;
+}
+
+// Light words of one visible face (dense index i of the current group); sets its transparent-pass bit.
+template <int NT> __device__ __forceinline__ void face_key(Smem<NT> &s, int i) {
+	const uint32_t d = s.fs.desc[i];
+	const int t = d & 127, u = (d >> 7) & 15, v = (d >> 11) & 15;
+	const int A = t / 34, sl = (t % 34) >> 1, inv = t & 1, face = 2 * A + inv;
+	const int layer = inv ? sl : sl - 1;
+	const int self = (layer + 1) * axis_stride(A) + (u + 1) * axis_stride(u_axis(A)) + (v + 1) * axis_stride(v_axis(A));
+	const uint2 l = light4(s.vox, face, self);
+	const uint32_t tc = row_code(s, s.vox[self] >> kVoxPidShift, face);
+	s.fs.key[i] = make_uint2(l.x, l.y | (tc << 24));
+	if ((s.codes[tc] >> 16) & kCodeTransPass)
+		atomicOr(reinterpret_cast<uint32_t *>(&s.tp[t][u & ~1]), 1u << (v + 16 * (u & 1)));
+}
+
+// Equal-to-right and equal-to-below bits of visible face i (its neighbours' dense indices follow from
+// the visible masks, since faces are numbered row by row).
+template <int NT> __device__ __forceinline__ void face_links(Smem<NT> &s, int i) {
+	const uint32_t d = s.fs.desc[i];
+	const int t = d & 127, u = (d >> 7) & 15, v = (d >> 11) & 15;
+	const uint32_t ne = s.ne[t][u];
+	const uint2 k = s.fs.key[i];
+	if (v < 15 && ((ne >> (v + 1)) & 1u)) {
+		const uint2 kr = s.fs.key[i + 1];
+		if (kr.x == k.x && kr.y == k.y)
+			atomicOr(reinterpret_cast<uint32_t *>(&s.eh[t][u & ~1]), 1u << (v + 16 * (u & 1)));
+	}
+	if (u < 15) {
+		const uint32_t nd = s.ne[t][u + 1];
+		if ((nd >> v) & 1u) {
+			const int j = i + 1 + __popc(ne >> (v + 1)) + __popc(nd & ((1u << v) - 1u));
+			const uint2 kd = s.fs.key[j];
+			if (kd.x == k.x && kd.y == k.y)
+				atomicOr(reinterpret_cast<uint32_t *>(&s.ev[t][u & ~1]), 1u << (v + 16 * (u & 1)));
+		}
+	}
+}
+
+// ------------------------------------------------------------------------------------------------
+// greedy merge of one task (:357-476), one lane per task
+// ------------------------------------------------------------------------------------------------
+// The reference scans rows u ascending and cells v ascending; at an uncovered visible cell it grows the
+// quad along V while the next cell has an equal face word and is uncovered, then along U while every
+// cell of the next row's segment is equal and uncovered, and marks the covered cells. The quads that start
+// in row u are therefore exactly the maximal runs of uncovered visible cells split wherever the face word
+// changes, and each run's height depends only on the rows below. Here a row's runs are found with bit
+// operations and all of them are extended downward together.
+struct Quad {
+	uint32_t u, v0, w, h, trans;
+	uint32_t ord, ord_pass; // index within the task, and within the task's quads of the same pass
+};
+
+// Moves marked bits from v to v - 1 (spread_down) or v - 1 to v (spread_up) while v is in `link`, i.e.
+// while v continues the run of v - 1: it reaches a run's start from any of its cells, and vice versa.
+__device__ __forceinline__ uint32_t spread_down(uint32_t d, uint32_t link) {
+	uint32_t m = link;
+	d |= (d & m) >> 1;
+	m &= m >> 1;
+	d |= (d & (m << 1)) >> 2;
+	m &= m >> 2;
+	d |= (d & (m << 3)) >> 4;
+	m &= m >> 4;
+	d |= (d & (m << 7)) >> 8;
+	return d;
+}
+__device__ __forceinline__ uint32_t spread_up(uint32_t f, uint32_t link) {
+	uint32_t m = link;
+	f |= (f << 1) & m;
+	m &= m << 1;
+	f |= (f << 2) & m;
+	m &= m << 2;
+	f |= (f << 4) & m;
+	m &= m << 4;
+	f |= (f << 8) & m;
+	return f;
+}
+
+template <int NT, bool EMIT, typename Fn>
+__device__ __forceinline__ void greedy_task(const Smem<NT> &s, int t, bool active, uint32_t &n_o, uint32_t &n_t,
+                                            Fn &&emit) {
+	// coverage of the rows below the current one by quads of earlier rows, and the equal-to-below masks,
+	// held in registers as windows that shift up by one row per step (window row k = row u + k)
+	uint32_t cov[8], ev[8];
+	const uint32_t *evs = reinterpret_cast<const uint32_t *>(s.ev[active ? t : 0]);
+#pragma unroll
+	for (int k = 0; k < 8; ++k) {
+		cov[k] = 0;
+		ev[k] = active ? evs[k] : 0u;
+	}
+	n_o = 0, n_t = 0;
+	for (int u = 0; u < 16; ++u) {
+		const uint32_t ne = active ? s.ne[t][u] : 0u, eh = active ? s.eh[t][u] : 0u, tp = active ? s.tp[t][u] : 0u;
+		const uint32_t S = ne & ~(cov[0] & 0xffffu); // uncovered visible cells of row u
+		const uint32_t link = S & ((S & eh) << 1);   // v continues the run of v - 1
+		const uint32_t starts = S & ~link;
+		if (__ballot(S != 0)) {
+			uint32_t alive = starts, cols = S;
+#pragma unroll
+			for (int k = 1; k <= 16; ++k) {
+				// row u + k, if it exists: every column of a surviving run must continue into it
+				if (!__ballot(alive != 0))
+					break;
+				uint32_t ok = 0;
+				if (k < 16) {
+					const uint32_t evr = (ev[(k - 1) >> 1] >> (16 * ((k - 1) & 1))) & 0xffffu;
+					const uint32_t cvr = (cov[k >> 1] >> (16 * (k & 1))) & 0xffffu;
+					ok = u + k < 16 ? (evr & ~cvr) : 0u;
+				}
+				const uint32_t bad = cols & ~ok;
+				uint32_t dead = 0;
+				if (bad) {
+					dead = spread_down(bad, link) & alive;
+					alive &= ~dead;
+					cols &= ~spread_up(dead, link);
+				}
+				if (EMIT) {
+					while (dead) {
+						const uint32_t v0 = __ffs(dead) - 1;
+						dead &= dead - 1;
+						const uint32_t below = (1u << v0) - 1u;
+						const uint32_t tr = (tp >> v0) & 1u;
+						Quad q;
+						q.u = u, q.v0 = v0, q.h = k, q.trans = tr;
+						q.w = __ffs(~(link >> (v0 + 1)));
+						q.ord = n_o + n_t + __popc(starts & below);
+						q.ord_pass = tr ? n_t + __popc(starts & tp & below) : n_o + __popc(starts & ~tp & below);
+						emit(q);
+					}
+				}
+				if (k < 16)
+					cov[k >> 1] |= cols << (16 * (k & 1));
+			}
+			const uint32_t nt = __popc(starts & tp);
+			n_t += nt;
+			n_o += __popc(starts) - nt;
+		}
+		// shift the windows up by one row
+#pragma unroll
+		for (int k = 0; k < 8; ++k) {
+			const uint32_t nc = k < 7 ? cov[k + 1] : 0u, nv = k < 7 ? ev[k + 1] : 0u;
+			cov[k] = (cov[k] >> 16) | (nc << 16);
+			ev[k] = (ev[k] >> 16) | (nv << 16);
+		}
+	}
 }
 
 // ------------------------------------------------------------------------------------------------
 // one chunk
 // ------------------------------------------------------------------------------------------------
 template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p, Smem<NT> &s, uint32_t job) {
-	const int tid = threadIdx.x;
+	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 	constexpr int CPT = HCM_CHUNK_VOXELS / NT; // custom cells per thread
 	if (tid < 27)
 		s.slots[tid] = p.nbr[27 * (size_t)job + tid];
@@ -724,39 +764,60 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 		a[tid] = (tid % 6) < 3 ? 0xffffffffu : 0u; // AABB<uint32_t>{}: min = max(), max = min() (dep/AABB.hpp:11)
 	}
 	if (tid == 0)
-		s.n_events = 0;
+		s.n_events = 0, s.flags = 0;
 	__syncthreads();
 
-	// 1. halo staging
+	// 1. halo staging: one thread per x-row (hy, hz) of 18 voxels
 	const uint16_t ref_block = p.blocks[(size_t)s.slots[26] * HCM_CHUNK_VOXELS];
 	uint32_t my_flags = 0;
-	for (int i = tid; i < kHaloVox; i += NT) {
-		const int hx = i % kHalo, rest = i / kHalo, hz = rest % kHalo, hy = rest / kHalo;
-		const int x = hx - 1, y = hy - 1, z = hz - 1;
-		const size_t g = (size_t)s.slots[slot27(x, y, z)] * HCM_CHUNK_VOXELS + local16(x, y, z);
-		const uint16_t b = p.blocks[g];
-		const uint32_t lt = p.light[g];
-		const uint32_t pid = p.reg.pid_of[b];
-		const uint32_t rf = (s.rows[pid].y >> 16) & 0xffu;
-		s.vox[i] = lt | ((rf & kRowDirectPass) << 8) | (((rf & kRowIndirectPass) >> 1) << 9) | (pid << 16);
-		if (b != ref_block)
-			my_flags |= kFlagNonUniform;
-		if (rf & kRowHasMesh) {
-			const bool inside = (unsigned)x < 16u && (unsigned)y < 16u && (unsigned)z < 16u;
-			if (inside) {
-				const uint32_t cell = local16(x, y, z);
-				atomicOr(&s.custom_bits[cell >> 5], 1u << (cell & 31));
-				my_flags |= kFlagCustom;
-			}
-			if (rf & kRowLiquidFlow) {
-				atomicOr(&s.flow_bits[i >> 5], 1u << (i & 31));
-				my_flags |= kFlagFlow;
+	for (int row = tid; row < kHaloRows; row += NT) {
+		const int hz = row % kHalo, hy = row / kHalo, y = hy - 1, z = hz - 1;
+		const size_t lrow = 16u * (uint32_t(z & 15) + 16u * uint32_t(y & 15));
+		const size_t cm = (size_t)s.slots[slot27(0, y, z)] * HCM_CHUNK_VOXELS + lrow;
+		const size_t cl = (size_t)s.slots[slot27(-1, y, z)] * HCM_CHUNK_VOXELS + lrow + 15;
+		const size_t cr = (size_t)s.slots[slot27(16, y, z)] * HCM_CHUNK_VOXELS + lrow;
+		const uint4 b0 = *reinterpret_cast<const uint4 *>(p.blocks + cm);
+		const uint4 b1 = *reinterpret_cast<const uint4 *>(p.blocks + cm + 8);
+		const uint4 lm = *reinterpret_cast<const uint4 *>(p.light + cm);
+		const uint32_t bl = p.blocks[cl], br = p.blocks[cr], ll = p.light[cl], lr = p.light[cr];
+		const uint32_t bw[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
+		const uint32_t lw[4] = {lm.x, lm.y, lm.z, lm.w};
+		uint32_t blk[18], lt[18];
+		blk[0] = bl, lt[0] = ll, blk[17] = br, lt[17] = lr;
+#pragma unroll
+		for (int x = 0; x < 16; ++x) {
+			blk[x + 1] = (bw[x >> 1] >> (16 * (x & 1))) & 0xffffu;
+			lt[x + 1] = (lw[x >> 2] >> (8 * (x & 3))) & 0xffu;
+		}
+		uint32_t pids[18];
+#pragma unroll
+		for (int x = 0; x < 18; ++x)
+			pids[x] = p.reg.pid_of[blk[x]];
+		const bool inside_yz = (unsigned)y < 16u && (unsigned)z < 16u;
+		uint32_t mesh_bits = 0;
+		uint32_t *dst = s.vox + row * kHalo;
+#pragma unroll
+		for (int x = 0; x < 18; ++x) {
+			const uint32_t pv = s.pid_vox[pids[x]];
+			dst[x] = lt[x] | pv;
+			if (blk[x] != ref_block)
+				my_flags |= kFlagNonUniform;
+			if (pv & kVoxMesh) {
+				if (inside_yz && x >= 1 && x <= 16)
+					mesh_bits |= 1u << (x - 1);
+				if (pv & kVoxFlow) {
+					const int i = row * kHalo + x;
+					atomicOr(&s.flow_bits[i >> 5], 1u << (i & 31));
+					my_flags |= kFlagFlow;
+				}
 			}
 		}
+		if (mesh_bits) {
+			const uint32_t cell0 = 16u * (uint32_t(z) + 16u * uint32_t(y));
+			atomicOr(&s.custom_bits[cell0 >> 5], mesh_bits << (cell0 & 31));
+			my_flags |= kFlagCustom;
+		}
 	}
-	if (tid == 0)
-		s.flags = 0;
-	__syncthreads();
 	if (my_flags)
 		atomicOr(&s.flags, my_flags);
 	__syncthreads();
@@ -768,25 +829,89 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 		return;
 	}
 
-	// 2. liquid-flow cells override their face textures with the flow mesh's textures (:221-236)
+	// 2. liquid-flow cells take the flow mesh's face textures (:221-236)
 	if (flags & kFlagFlow) {
+		constexpr uint32_t kKeep = 0xffu | kVoxDirect | kVoxIndirect;
 		for (int i = tid; i < kHaloVox; i += NT) {
 			if (!((s.flow_bits[i >> 5] >> (i & 31)) & 1u))
 				continue;
 			const int hx = i % kHalo, rest = i / kHalo, hz = rest % kHalo, hy = rest / kHalo;
-			const int x = hx - 1, y = hy - 1, z = hz - 1;
-			const uint32_t pid = s.vox[i] >> 16;
+			const uint32_t pid = s.vox[i] >> kVoxPidShift;
 			const uint32_t mesh = s.rows[pid].y >> 24;
-			const Flow f = flow_heights(p, s.slots, x, y, z, p.reg.meshes[mesh].flow_height);
-			s.vox[i] = (s.vox[i] & 0xffffu) | ((p.reg.flow_pid_base + flow_pattern(f)) << 16);
+			const Flow f = flow_heights(p, s.slots, hx - 1, hy - 1, hz - 1, p.reg.meshes[mesh].flow_height);
+			const uint32_t tmpl = s.pid_vox[p.reg.flow_pid_base + flow_pattern(f)];
+			s.vox[i] = (s.vox[i] & kKeep) | (tmpl & ~kKeep);
 		}
 		__syncthreads();
 	}
 
-	// 3. visible faces, light words and merge masks
-	face_tasks(s);
+	// 3. visibility masks; zero the link masks
+	for (int t = wave; t < kTasks; t += NT / 64) {
+		const uint32_t n = task_visibility(s, t);
+		if (lane == 0)
+			s.tpre[t] = n;
+	}
+	for (int i = tid; i < kTasks * 8; i += NT) {
+		reinterpret_cast<uint32_t *>(s.eh)[i] = 0;
+		reinterpret_cast<uint32_t *>(s.ev)[i] = 0;
+		reinterpret_cast<uint32_t *>(s.tp)[i] = 0;
+	}
+	__syncthreads();
+	if (wave == 0) { // exclusive prefix of visible faces over the tasks
+		const uint32_t a = lane < kTasks ? s.tpre[lane] : 0u, b = lane + 64 < kTasks ? s.tpre[lane + 64] : 0u;
+		uint32_t ia = a, ib = b;
+#pragma unroll
+		for (int d = 1; d < 64; d <<= 1) {
+			const uint32_t xa = __shfl_up(ia, d), xb = __shfl_up(ib, d);
+			if (lane >= d)
+				ia += xa, ib += xb;
+		}
+		const uint32_t tot_a = __shfl(ia, 63);
+		if (lane < kTasks)
+			s.tpre[lane] = ia - a;
+		if (lane + 64 < kTasks)
+			s.tpre[lane + 64] = tot_a + ib - b;
+		if (lane == 63)
+			s.tpre[kTasks] = tot_a + ib;
+	}
+	__syncthreads();
 
-	// 4. counts: custom faces per thread range of cells, greedy quads per task
+	// 4. light words for the visible faces, a group of whole tasks at a time
+	for (int t0 = 0; t0 < kTasks;) {
+		if (tid == 0) { // the longest run of tasks whose faces fit the stage (one task always fits)
+			int t1 = t0 + 1;
+			while (t1 < kTasks && s.tpre[t1 + 1] - s.tpre[t0] <= (uint32_t)kFaceCap)
+				++t1;
+			s.group_end = t1;
+		}
+		__syncthreads();
+		const int t1 = s.group_end;
+		const uint32_t base = s.tpre[t0], n = s.tpre[t1] - base;
+		for (int pr = tid; pr < (t1 - t0) * 16; pr += NT) { // number the faces row by row
+			const int t = t0 + pr / 16, u = pr % 16;
+			uint32_t bits = s.ne[t][u];
+			if (!bits)
+				continue;
+			uint32_t at = s.tpre[t] - base;
+			for (int k = 0; k < u; ++k)
+				at += __popc(s.ne[t][k]);
+			while (bits) {
+				const uint32_t v = __ffs(bits) - 1;
+				bits &= bits - 1;
+				s.fs.desc[at++] = uint32_t(t) | uint32_t(u) << 7 | v << 11;
+			}
+		}
+		__syncthreads();
+		for (uint32_t i = tid; i < n; i += NT)
+			face_key(s, i);
+		__syncthreads();
+		for (uint32_t i = tid; i < n; i += NT)
+			face_links(s, i);
+		__syncthreads();
+		t0 = t1;
+	}
+
+	// 5. counts: custom faces per thread range of cells, greedy quads per task
 	uint32_t c_o = 0, c_t = 0;
 	if (flags & kFlagCustom) {
 		for (int k = 0; k < CPT; ++k) {
@@ -799,10 +924,9 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 			c_o += c.count - nt;
 		}
 	}
-	__syncthreads(); // masks complete
 	uint32_t q_o = 0, q_t = 0;
-	if (tid < kTasks)
-		greedy(s, tid, [&](const Quad &q) { (q.trans ? q_t : q_o) += 1; });
+	if (wave * 64 < kTasks)
+		greedy_task<NT, false>(s, tid, tid < kTasks, q_o, q_t, [](const Quad &) {});
 	uint64_t cust_total, task_total;
 	const uint64_t cust_pre = block_excl_scan(s, (uint64_t)c_o | ((uint64_t)c_t << 32), &cust_total);
 	const uint64_t task_pre = block_excl_scan(s, (uint64_t)q_o | ((uint64_t)q_t << 32), &task_total);
@@ -815,18 +939,18 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 		return;
 	}
 
-	// 5. allocation: the chunk's quads [pool_base, +n0) opaque, [+n0, +n0+n1) transparent
+	// 6. allocation: the chunk's quads [pool_base, +n0) opaque, [+n0, +n0+n1) transparent
 	if (tid == 0) {
 		const uint32_t nm0 = (n0 + kSplitQuads - 1) / kSplitQuads, nm1 = (n1 + kSplitQuads - 1) / kSplitQuads;
 		uint32_t status = HCM_OK;
-		const uint32_t base = atomicAdd(&p.ctr->pool_cursor, total);
-		if ((uint64_t)base + total > p.pool_cap)
+		const uint32_t pbase = atomicAdd(&p.ctr->pool_cursor, total);
+		if ((uint64_t)pbase + total > p.pool_cap)
 			status = (uint32_t)HCM_ERR_POOL_FULL;
 		const uint32_t mbase = atomicAdd(&p.ctr->mesh_cursor, nm0 + nm1);
 		if ((uint64_t)mbase + nm0 + nm1 > p.rec_cap)
 			status = (uint32_t)HCM_ERR_POOL_FULL;
 		s.status = status;
-		s.pool_base = base;
+		s.pool_base = pbase;
 		s.mesh_base = mbase;
 		s.n_mesh0 = nm0;
 		s.n_mesh1 = nm1;
@@ -838,7 +962,7 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 		return;
 	const uint32_t pool_base = s.pool_base;
 
-	// 6. describe quads into the LDS list window by window, then write them one lane per quad
+	// 7. describe quads into the LDS list window by window, then write them one lane per quad
 	const uint32_t task_o = cust0 + (uint32_t)task_pre, task_t = cust1 + (uint32_t)(task_pre >> 32);
 	const uint32_t task_g = cust0 + cust1 + (uint32_t)task_pre + (uint32_t)(task_pre >> 32);
 	for (uint32_t w0 = 0; w0 < total; w0 += kListCap) {
@@ -859,18 +983,18 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 				}
 			}
 		}
-		if (tid < kTasks) {
-			uint32_t io = task_o, it = task_t, g = task_g;
-			const uint32_t my_end = task_g + q_o + q_t;
-			if (g < w1 && my_end > w0)
-				greedy(s, tid, [&](const Quad &q) {
-					const uint32_t i = q.trans ? it++ : io++;
-					if (g >= w0 && g < w1)
-						s.list[g - w0] = make_uint2(uint32_t(tid) | (uint32_t(q.u) << 7) | (uint32_t(q.v0) << 11) |
-						                                (uint32_t(q.w - 1) << 15) | (uint32_t(q.h - 1) << 19),
-						                            i | (uint32_t(q.trans) << 31));
-					++g;
-				});
+		if (wave * 64 < kTasks) {
+			const bool mine = tid < kTasks && task_g < w1 && task_g + q_o + q_t > w0;
+			uint32_t eo, et;
+			greedy_task<NT, true>(s, tid, mine, eo, et, [&](const Quad &q) {
+				const uint32_t g = task_g + q.ord;
+				if (g >= w0 && g < w1) {
+					const uint32_t i = (q.trans ? task_t : task_o) + q.ord_pass;
+					s.list[g - w0] = make_uint2(uint32_t(tid) | (q.u << 7) | (q.v0 << 11) | ((q.w - 1) << 15) |
+					                                ((q.h - 1) << 19),
+					                            i | (q.trans << 31));
+				}
+			});
 		}
 		__syncthreads();
 		const uint32_t n = min(total - w0, (uint32_t)kListCap);
@@ -893,14 +1017,14 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 		__syncthreads();
 	}
 
-	// 7. mesh records in the reference's order: split meshes by time, then opaque, then transparent
+	// 8. mesh records in the reference's order: split meshes by time, then opaque, then transparent
 	if (tid == 0) {
 		const uint32_t ne = min(s.n_events, 16u);
 		for (uint32_t a = 1; a < ne; ++a) // insertion sort by emission position
 			for (uint32_t b = a; b > 0 && s.events[b - 1][0] > s.events[b][0]; --b) {
-				uint32_t t0 = s.events[b][0], t1 = s.events[b][1];
+				const uint32_t e0 = s.events[b][0], e1 = s.events[b][1];
 				s.events[b][0] = s.events[b - 1][0], s.events[b][1] = s.events[b - 1][1];
-				s.events[b - 1][0] = t0, s.events[b - 1][1] = t1;
+				s.events[b - 1][0] = e0, s.events[b - 1][1] = e1;
 			}
 		const uint32_t nm0 = s.n_mesh0, nm1 = s.n_mesh1, mbase = s.mesh_base;
 		uint32_t m = 0;
@@ -927,8 +1051,11 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 
 template <int NT> __global__ __launch_bounds__(NT) void mesh_kernel(MeshParams p) {
 	__shared__ Smem<NT> s;
-	for (int i = threadIdx.x; i < kMaxPid; i += NT)
-		s.rows[i] = i < (int)p.reg.n_pid ? p.reg.rows[i] : make_uint2(0, 0);
+	for (int i = threadIdx.x; i < kMaxPid; i += NT) {
+		const bool ok = i < (int)p.reg.n_pid;
+		s.rows[i] = ok ? p.reg.rows[i] : make_uint2(0, 0);
+		s.pid_vox[i] = ok ? p.reg.pid_vox[i] : (uint32_t(i) << kVoxPidShift);
+	}
 	for (int i = threadIdx.x; i < kMaxCodes; i += NT)
 		s.codes[i] = i < (int)p.reg.n_codes ? p.reg.codes[i] : (kCodeEmpty | kCodeTransparent) << 16;
 	if (!p.persistent) {
@@ -957,6 +1084,25 @@ __global__ void reset_kernel(Counters *ctr) {
 		ctr->mesh_cursor = 0;
 		ctr->pad = 0;
 	}
+}
+
+// Applies voxel edits in order of their index; edits to one voxel within a launch are resolved by the
+// host, which keeps only the last one.
+__global__ void edit_kernel(uint16_t *blocks, uint8_t *light, const VoxelEdit *edits, uint32_t n) {
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) {
+		const VoxelEdit e = edits[i];
+		const size_t g = (size_t)e.slot * HCM_CHUNK_VOXELS + e.local;
+		blocks[g] = uint16_t(e.block);
+		light[g] = uint8_t(e.light);
+	}
+}
+
+hipError_t launch_edits(uint16_t *blocks, uint8_t *light, const VoxelEdit *edits, uint32_t n, hipStream_t stream) {
+	if (n == 0)
+		return hipSuccess;
+	hipLaunchKernelGGL(edit_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, blocks, light, edits, n);
+	return hipGetLastError();
 }
 
 hipError_t launch_reset(Counters *ctr, hipStream_t stream) {

@@ -1,0 +1,892 @@
+// hcm_serve.cpp -- continuous batching of chunk-load and block-edit requests (include/hcm.h, "Serving").
+//
+// The world's chunks live in device slots. Callers submit requests from any thread; one scheduler thread
+// applies them to the world model in arrival order (request k gets epoch k), keeps the set of chunks whose
+// mesh is stale, and, whenever fewer than max_inflight batches are on the device, forms a batch from what
+// is pending:
+//   copy stream:    first-time chunk loads -> their fresh slots (no running batch reads a fresh slot);
+//   compute stream: wait for those copies; reloads of resident chunks (in stream order, after every
+//                   earlier batch that may read them); the batch's voxel edits; then up to max_batch stale
+//                   chunks meshed by the persistent kernel into an arena taken from the vertex pool, and
+//                   the per-chunk results copied back.
+// Uploads of batch i+1 overlap the meshing of batch i. A batch meshes chunks only when every pending data
+// change has been flushed into it, so its meshes reflect all requests up to the epoch at its launch; a
+// batch that cannot flush everything (upload or edit limits) carries data only, and its "applied epoch"
+// is the last epoch before the first change it left pending. A request completes when a finished batch
+// has applied it and every chunk it made stale holds a mesh from a batch at or after its epoch.
+// Remesh rules follow the reference: a chunk is meshed only when all 26 neighbours are present
+// (client/src/ChunkMeshTask.cpp:14-39), and an edit invalidates every chunk whose 18^3 halo holds the
+// voxel -- the edited chunk plus the face/edge/corner neighbours the voxel touches.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "hcm_internal.h"
+
+using namespace hcm;
+
+#define HCM_CHECK(call)                                                                                            \
+	do {                                                                                                           \
+		if ((call) != hipSuccess)                                                                                  \
+			return HCM_ERR_HIP;                                                                                    \
+	} while (0)
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+inline uint64_t chunk_key(int32_t cx, int32_t cy, int32_t cz) {
+	return (uint64_t(uint32_t(cx) & 0x1fffffu) << 42) | (uint64_t(uint32_t(cy) & 0x1fffffu) << 21) |
+	       uint64_t(uint32_t(cz) & 0x1fffffu);
+}
+inline int32_t sext21(uint64_t v) { return int32_t(uint32_t(v << 11)) >> 11; }
+inline void key_pos(uint64_t k, int32_t p[3]) {
+	p[0] = sext21(k >> 42 & 0x1fffffu), p[1] = sext21(k >> 21 & 0x1fffffu), p[2] = sext21(k & 0x1fffffu);
+}
+inline int32_t floor_div16(int32_t v) { return v >= 0 ? v / 16 : -((-v + 15) / 16); }
+
+// neighbour slot -> chunk offset (common/include/common/Position.hpp:146-152)
+constexpr int kRev[3] = {1, -1, 0};
+inline void slot_offset(int s, int d[3]) { d[0] = kRev[s / 9], d[1] = kRev[(s / 3) % 3], d[2] = kRev[s % 3]; }
+
+constexpr uint32_t kMaxUploads = 1024;  // chunk uploads staged per batch
+constexpr uint32_t kMaxEdits = 65536;   // voxel edits applied per batch
+constexpr uint64_t kMinArena = 1 << 16; // quads; more than any one chunk's meshes
+
+struct Range {
+	uint64_t first = 0, count = 0;
+};
+
+// First-fit allocator of quad ranges in [0, capacity).
+class RangeAllocator {
+public:
+	explicit RangeAllocator(uint64_t cap) : m_cap(cap), m_free_total(cap) {
+		if (cap)
+			m_free[0] = cap;
+	}
+	bool alloc(uint64_t n, uint64_t *first) {
+		for (auto it = m_free.begin(); it != m_free.end(); ++it)
+			if (it->second >= n) {
+				*first = it->first;
+				const uint64_t rest = it->second - n, at = it->first + n;
+				m_free.erase(it);
+				if (rest)
+					m_free[at] = rest;
+				m_free_total -= n;
+				return true;
+			}
+		return false;
+	}
+	void release(uint64_t first, uint64_t n) {
+		if (!n)
+			return;
+		m_free_total += n;
+		auto it = m_free.emplace(first, n).first;
+		auto nx = std::next(it);
+		if (nx != m_free.end() && it->first + it->second == nx->first) {
+			it->second += nx->second;
+			m_free.erase(nx);
+		}
+		if (it != m_free.begin()) {
+			auto pv = std::prev(it);
+			if (pv->first + pv->second == it->first) {
+				pv->second += it->second;
+				m_free.erase(it);
+			}
+		}
+	}
+	uint64_t largest() const {
+		uint64_t m = 0;
+		for (auto &f : m_free)
+			m = std::max(m, f.second);
+		return m;
+	}
+	uint64_t free_total() const { return m_free_total; }
+	uint64_t capacity() const { return m_cap; }
+
+private:
+	uint64_t m_cap, m_free_total;
+	std::map<uint64_t, uint64_t> m_free; // first -> count
+};
+
+struct Chunk {
+	uint32_t slot = 0;
+	bool stale = false;             // mesh out of date and not in a launched batch
+	bool has_mesh = false;
+	uint64_t meshed_epoch = 0;      // epoch of the batch that produced the current meshes
+	Range mesh;                     // their pool range
+	std::vector<hcm_mesh_rec> recs; // their records (absolute first_quad)
+};
+
+struct Request {
+	uint64_t tag = 0;
+	uint32_t kind = 0;
+	Clock::time_point t_submit;
+	int32_t pos[3] = {0, 0, 0}; // chunk (load) or world voxel (edit)
+	std::vector<uint16_t> blocks;
+	std::vector<uint8_t> light;
+	uint16_t block = 0;
+	uint8_t light1 = 0;
+	uint32_t slot = 0;             // load: the chunk's device slot
+	uint64_t epoch = 0;            // set when the scheduler applies it
+	std::vector<uint64_t> targets; // chunks it made stale
+};
+
+struct Upload {
+	uint32_t slot;
+	bool reload; // the slot already holds the chunk: copy in compute-stream order
+	Request *req;
+};
+
+struct PendingEdit {
+	VoxelEdit e;
+	uint64_t first_epoch; // earliest request still pending for this voxel
+};
+
+struct BatchSlot { // device and pinned buffers of one in-flight batch
+	uint32_t *d_nbr = nullptr;
+	VoxelEdit *d_edits = nullptr;
+	Counters *d_ctr = nullptr;
+	hcm_job_result *d_res = nullptr;
+	hcm_mesh_rec *d_recs = nullptr;
+	uint32_t *h_nbr = nullptr;
+	VoxelEdit *h_edits = nullptr;
+	uint16_t *h_blocks = nullptr;
+	uint8_t *h_light = nullptr;
+	hcm_job_result *h_res = nullptr;
+	hcm_mesh_rec *h_recs = nullptr;
+	Counters *h_ctr = nullptr;
+	hipEvent_t ev_up = nullptr, ev_k0 = nullptr, ev_k1 = nullptr, ev_done = nullptr;
+	bool busy = false;
+	bool arena_was_largest = false; // the arena took the largest free range there was
+	uint64_t id = 0, epoch = 0;     // epoch: every request up to it is applied on the device
+	std::vector<uint64_t> jobs;
+	Range arena;
+};
+
+struct HistoryEntry {
+	int32_t pos[3];
+	uint64_t epoch;
+	std::vector<hcm_mesh_rec> recs; // first_quad relative to verts/idx below
+	std::vector<uint2> verts;
+	std::vector<uint16_t> idx;
+};
+
+} // namespace
+
+struct hcm_server {
+	hcm_ctx *ctx = nullptr;
+	hcm_server_config cfg{};
+	CtxInfo info{};
+	hipStream_t s_copy = nullptr, s_comp = nullptr;
+	uint16_t *d_blocks = nullptr;
+	uint8_t *d_light = nullptr;
+	uint2 *d_verts = nullptr;
+	uint16_t *d_idx = nullptr;
+	std::vector<BatchSlot> slots;
+	std::atomic<int> error{HCM_OK};
+
+	// submission side (in_mu)
+	std::mutex in_mu;
+	std::condition_variable in_cv;
+	std::deque<std::unique_ptr<Request>> incoming;
+	std::unordered_map<uint64_t, uint32_t> accepted; // chunk -> slot, fixed at the first accepted load
+	uint64_t submitted = 0;
+	bool stop = false;
+
+	// scheduler state: changed by the scheduler thread under world_mu
+	std::mutex world_mu;
+	std::unordered_map<uint64_t, Chunk> world;
+	std::deque<uint64_t> stale_fifo;
+	std::vector<std::unique_ptr<Request>> waiting;
+	std::deque<Upload> uploads;
+	std::map<uint64_t, PendingEdit> edits; // (slot << 12 | local) -> last edit of that voxel
+	uint64_t epoch = 0, batches = 0, applied_done = 0;
+	RangeAllocator *pool = nullptr;
+	double quads_per_chunk = 512.0; // running estimate that sizes arenas
+
+	// completions (out_mu)
+	std::mutex out_mu;
+	std::condition_variable out_cv;
+	std::deque<hcm_completion> done;
+	uint64_t completed = 0;
+
+	// stats and history (st_mu)
+	std::mutex st_mu;
+	hcm_server_stats stats{};
+	std::vector<HistoryEntry> history;
+	std::thread thread;
+
+	void fail(int code) {
+		error = code;
+		std::lock_guard<std::mutex> lk(out_mu);
+		out_cv.notify_all();
+	}
+
+	bool meshable(const int32_t p[3]) const {
+		for (int s = 0; s < 27; ++s) {
+			int d[3];
+			slot_offset(s, d);
+			if (!world.count(chunk_key(p[0] + d[0], p[1] + d[1], p[2] + d[2])))
+				return false;
+		}
+		return true;
+	}
+	void mark_stale(int32_t cx, int32_t cy, int32_t cz, std::vector<uint64_t> &targets) {
+		const int32_t p[3] = {cx, cy, cz};
+		const uint64_t key = chunk_key(cx, cy, cz);
+		auto it = world.find(key);
+		if (it == world.end() || !meshable(p))
+			return;
+		if (!it->second.stale) {
+			it->second.stale = true;
+			stale_fifo.push_back(key);
+		}
+		targets.push_back(key);
+	}
+
+	void apply(std::unique_ptr<Request> r) {
+		r->epoch = ++epoch;
+		if (r->kind == HCM_REQ_LOAD) {
+			const uint64_t key = chunk_key(r->pos[0], r->pos[1], r->pos[2]);
+			auto it = world.find(key);
+			const bool reload = it != world.end();
+			const uint32_t slot = r->slot;
+			if (!reload) {
+				Chunk c;
+				c.slot = slot;
+				world.emplace(key, c);
+			}
+			// pending edits of the chunk are superseded by the new data
+			auto e = edits.lower_bound(uint64_t(slot) << 12);
+			while (e != edits.end() && (e->first >> 12) == slot)
+				e = edits.erase(e);
+			uploads.push_back(Upload{slot, reload, r.get()});
+			for (int s = 0; s < 27; ++s) {
+				int d[3];
+				slot_offset(s, d);
+				mark_stale(r->pos[0] + d[0], r->pos[1] + d[1], r->pos[2] + d[2], r->targets);
+			}
+		} else {
+			const int32_t cx = floor_div16(r->pos[0]), cy = floor_div16(r->pos[1]), cz = floor_div16(r->pos[2]);
+			const int32_t lx = r->pos[0] - 16 * cx, ly = r->pos[1] - 16 * cy, lz = r->pos[2] - 16 * cz;
+			const Chunk &c = world.at(chunk_key(cx, cy, cz));
+			const uint32_t local = uint32_t(lx) + 16u * (uint32_t(lz) + 16u * uint32_t(ly));
+			const uint64_t vkey = (uint64_t(c.slot) << 12) | local;
+			auto e = edits.find(vkey);
+			const VoxelEdit ve{c.slot, local, r->block, r->light1};
+			if (e == edits.end())
+				edits.emplace(vkey, PendingEdit{ve, r->epoch});
+			else
+				e->second.e = ve;
+			// a voxel on a chunk border lies in the halo of the neighbours across that border
+			const int ox[2] = {0, lx == 0 ? -1 : (lx == 15 ? 1 : 0)};
+			const int oy[2] = {0, ly == 0 ? -1 : (ly == 15 ? 1 : 0)};
+			const int oz[2] = {0, lz == 0 ? -1 : (lz == 15 ? 1 : 0)};
+			for (int a = 0; a < (ox[1] ? 2 : 1); ++a)
+				for (int b = 0; b < (oy[1] ? 2 : 1); ++b)
+					for (int d = 0; d < (oz[1] ? 2 : 1); ++d)
+						mark_stale(cx + ox[a], cy + oy[b], cz + oz[d], r->targets);
+		}
+		waiting.push_back(std::move(r));
+	}
+
+	bool any_busy() const {
+		for (auto &b : slots)
+			if (b.busy)
+				return true;
+		return false;
+	}
+	bool work_pending() const { return !stale_fifo.empty() || !uploads.empty() || !edits.empty() || any_busy(); }
+
+	// Returns 0 when launched, 1 when there is nothing to launch now, or an error.
+	int launch_batch(BatchSlot &b) {
+		const uint32_t n_up = (uint32_t)std::min<size_t>(uploads.size(), kMaxUploads);
+		const bool up_rest = uploads.size() > n_up;
+		const uint32_t n_ed = up_rest ? 0u : (uint32_t)std::min<size_t>(edits.size(), kMaxEdits);
+		const bool ed_rest = edits.size() > n_ed;
+		// meshing only once every pending change is in this batch
+		b.jobs.clear();
+		if (!up_rest && !ed_rest)
+			while (!stale_fifo.empty() && b.jobs.size() < cfg.max_batch) {
+				const uint64_t key = stale_fifo.front();
+				stale_fifo.pop_front();
+				auto it = world.find(key);
+				if (it != world.end() && it->second.stale) { // a key queued twice is taken once
+					it->second.stale = false;
+					b.jobs.push_back(key);
+				}
+			}
+		// arena admission by vertex-pool memory
+		b.arena = Range{};
+		b.arena_was_largest = false;
+		if (!b.jobs.empty()) {
+			const uint64_t want = std::min<uint64_t>(
+			    pool->capacity(), std::max<uint64_t>(kMinArena, uint64_t(quads_per_chunk * 1.25 * double(b.jobs.size()))));
+			const uint64_t largest = pool->largest();
+			uint64_t first = 0, got = want;
+			bool ok = pool->alloc(want, &first);
+			if (!ok && !any_busy()) { // nothing will free memory soon: take the largest free range
+				got = largest;
+				if (got == 0)
+					return HCM_ERR_POOL_FULL;
+				ok = pool->alloc(got, &first);
+			}
+			b.arena_was_largest = ok && got >= largest;
+			if (!ok) { // wait for running batches to return memory
+				for (auto it = b.jobs.rbegin(); it != b.jobs.rend(); ++it) {
+					world.at(*it).stale = true;
+					stale_fifo.push_front(*it);
+				}
+				b.jobs.clear();
+			} else
+				b.arena = Range{first, got};
+		}
+		if (b.jobs.empty() && n_up == 0 && n_ed == 0)
+			return 1;
+		uint64_t applied = epoch;
+		if (up_rest)
+			applied = std::min(applied, uploads[n_up].req->epoch - 1);
+		b.id = ++batches;
+		b.busy = true;
+
+		const size_t V = HCM_CHUNK_VOXELS;
+		for (uint32_t i = 0; i < n_up; ++i) {
+			Request *r = uploads[i].req;
+			std::memcpy(b.h_blocks + i * V, r->blocks.data(), V * 2);
+			std::memcpy(b.h_light + i * V, r->light.data(), V);
+			std::vector<uint16_t>().swap(r->blocks);
+			std::vector<uint8_t>().swap(r->light);
+		}
+		for (uint32_t i = 0; i < n_up; ++i)
+			if (!uploads[i].reload) {
+				const uint32_t slot = uploads[i].slot;
+				HCM_CHECK(hipMemcpyAsync(d_blocks + slot * V, b.h_blocks + i * V, V * 2, hipMemcpyHostToDevice, s_copy));
+				HCM_CHECK(hipMemcpyAsync(d_light + slot * V, b.h_light + i * V, V, hipMemcpyHostToDevice, s_copy));
+			}
+		HCM_CHECK(hipEventRecord(b.ev_up, s_copy));
+		HCM_CHECK(hipStreamWaitEvent(s_comp, b.ev_up, 0));
+		for (uint32_t i = 0; i < n_up; ++i)
+			if (uploads[i].reload) {
+				const uint32_t slot = uploads[i].slot;
+				HCM_CHECK(hipMemcpyAsync(d_blocks + slot * V, b.h_blocks + i * V, V * 2, hipMemcpyHostToDevice, s_comp));
+				HCM_CHECK(hipMemcpyAsync(d_light + slot * V, b.h_light + i * V, V, hipMemcpyHostToDevice, s_comp));
+			}
+		uploads.erase(uploads.begin(), uploads.begin() + n_up);
+
+		uint32_t k = 0;
+		for (auto it = edits.begin(); it != edits.end() && k < n_ed;) {
+			b.h_edits[k++] = it->second.e;
+			it = edits.erase(it);
+		}
+		for (auto &e : edits)
+			applied = std::min(applied, e.second.first_epoch - 1);
+		b.epoch = applied;
+		if (n_ed) {
+			HCM_CHECK(hipMemcpyAsync(b.d_edits, b.h_edits, n_ed * sizeof(VoxelEdit), hipMemcpyHostToDevice, s_comp));
+			HCM_CHECK(launch_edits(d_blocks, d_light, b.d_edits, n_ed, s_comp));
+		}
+
+		if (!b.jobs.empty()) {
+			for (size_t j = 0; j < b.jobs.size(); ++j) {
+				int32_t p[3];
+				key_pos(b.jobs[j], p);
+				for (int s = 0; s < 27; ++s) {
+					int d[3];
+					slot_offset(s, d);
+					b.h_nbr[27 * j + s] = world.at(chunk_key(p[0] + d[0], p[1] + d[1], p[2] + d[2])).slot;
+				}
+			}
+			HCM_CHECK(hipMemcpyAsync(b.d_nbr, b.h_nbr, b.jobs.size() * 27 * 4, hipMemcpyHostToDevice, s_comp));
+			HCM_CHECK(launch_reset(b.d_ctr, s_comp));
+			MeshParams p;
+			std::memset(&p, 0, sizeof(p));
+			p.reg = info.reg;
+			p.blocks = d_blocks;
+			p.light = d_light;
+			p.nbr = b.d_nbr;
+			p.n_jobs = (uint32_t)b.jobs.size();
+			p.persistent = 1;
+			p.pool_cap = (uint32_t)b.arena.count;
+			p.ctr = b.d_ctr;
+			p.pool_verts = d_verts + 4 * b.arena.first;
+			p.pool_idx = d_idx + 6 * b.arena.first;
+			p.recs = b.d_recs;
+			p.rec_cap = rec_cap();
+			p.results = b.d_res;
+			const int grid = (int)std::max<uint32_t>(1, std::min<uint32_t>(info.grid, (uint32_t)b.jobs.size()));
+			HCM_CHECK(hipEventRecord(b.ev_k0, s_comp));
+			HCM_CHECK(launch_mesh(p, (int)info.threads, grid, s_comp));
+			HCM_CHECK(hipEventRecord(b.ev_k1, s_comp));
+			HCM_CHECK(hipMemcpyAsync(b.h_res, b.d_res, b.jobs.size() * sizeof(hcm_job_result), hipMemcpyDeviceToHost,
+			                         s_comp));
+			HCM_CHECK(hipMemcpyAsync(b.h_ctr, b.d_ctr, sizeof(Counters), hipMemcpyDeviceToHost, s_comp));
+			HCM_CHECK(hipMemcpyAsync(b.h_recs, b.d_recs, rec_cap() * sizeof(hcm_mesh_rec), hipMemcpyDeviceToHost, s_comp));
+		}
+		HCM_CHECK(hipEventRecord(b.ev_done, s_comp));
+		return 0;
+	}
+
+	uint32_t rec_cap() const { return 3 * cfg.max_batch + 1024; }
+
+	int finish_batch(BatchSlot &b) {
+		if (!b.jobs.empty()) {
+			float ms = 0;
+			HCM_CHECK(hipEventElapsedTime(&ms, b.ev_k0, b.ev_k1));
+			std::vector<uint2> hv;
+			std::vector<uint16_t> hi;
+			const uint64_t used = std::min<uint64_t>(b.h_ctr->pool_cursor, b.arena.count);
+			if (cfg.keep_history && used) {
+				hv.resize(used * 4);
+				hi.resize(used * 6);
+				HCM_CHECK(hipMemcpy(hv.data(), d_verts + 4 * b.arena.first, used * 4 * sizeof(uint2), hipMemcpyDeviceToHost));
+				HCM_CHECK(hipMemcpy(hi.data(), d_idx + 6 * b.arena.first, used * 12, hipMemcpyDeviceToHost));
+			}
+			uint64_t meshed = 0, retries = 0, quads = 0;
+			std::vector<Range> kept; // arena ranges now holding live meshes
+			for (size_t j = 0; j < b.jobs.size(); ++j) {
+				const uint64_t key = b.jobs[j];
+				Chunk &c = world.at(key);
+				const hcm_job_result &res = b.h_res[j];
+				if (res.status != HCM_OK) { // arena or record space ran out: mesh it again later
+					++retries;
+					if (!c.stale) {
+						c.stale = true;
+						stale_fifo.push_back(key);
+					}
+					continue;
+				}
+				uint64_t lo = 0;
+				if (res.quad_count) { // one contiguous allocation per chunk
+					lo = ~0ull;
+					for (uint32_t m = 0; m < res.mesh_count; ++m)
+						lo = std::min<uint64_t>(lo, b.h_recs[res.first_mesh + m].first_quad);
+					kept.push_back(Range{lo, res.quad_count});
+				}
+				if (c.has_mesh)
+					pool->release(c.mesh.first, c.mesh.count);
+				c.has_mesh = true;
+				c.mesh = Range{b.arena.first + lo, res.quad_count};
+				c.recs.assign(b.h_recs + res.first_mesh, b.h_recs + res.first_mesh + res.mesh_count);
+				for (auto &r : c.recs)
+					r.first_quad += (uint32_t)b.arena.first;
+				c.meshed_epoch = b.epoch;
+				++meshed;
+				quads += res.quad_count;
+				if (cfg.keep_history) {
+					HistoryEntry h;
+					key_pos(key, h.pos);
+					h.epoch = b.epoch;
+					for (uint32_t m = 0; m < res.mesh_count; ++m) {
+						hcm_mesh_rec r = b.h_recs[res.first_mesh + m];
+						const size_t q0 = r.first_quad;
+						r.first_quad = (uint32_t)(h.verts.size() / 4);
+						h.verts.insert(h.verts.end(), hv.begin() + 4 * q0, hv.begin() + 4 * (q0 + r.quad_count));
+						h.idx.insert(h.idx.end(), hi.begin() + 6 * q0, hi.begin() + 6 * (q0 + r.quad_count));
+						h.recs.push_back(r);
+					}
+					std::lock_guard<std::mutex> lk(st_mu);
+					history.push_back(std::move(h));
+				}
+			}
+			// the parts of the arena no live mesh holds go back to the pool
+			std::sort(kept.begin(), kept.end(), [](const Range &x, const Range &y) { return x.first < y.first; });
+			uint64_t at = 0;
+			for (const Range &r : kept) {
+				pool->release(b.arena.first + at, r.first - at);
+				at = r.first + r.count;
+			}
+			pool->release(b.arena.first + at, b.arena.count - at);
+			if (meshed)
+				quads_per_chunk = 0.75 * quads_per_chunk + 0.25 * (double(quads) / double(meshed));
+			if (retries) {
+				quads_per_chunk *= 2.0;
+				if (!meshed && b.arena_was_largest)
+					return HCM_ERR_POOL_FULL; // not one chunk fits the largest free range
+			}
+			std::lock_guard<std::mutex> lk(st_mu);
+			stats.batches++;
+			stats.chunks_meshed += meshed;
+			stats.quads_meshed += quads;
+			stats.retries += retries;
+			stats.gpu_ms += ms;
+		}
+		applied_done = std::max(applied_done, b.epoch);
+		b.busy = false;
+		return HCM_OK;
+	}
+
+	void complete_ready() {
+		const auto now = Clock::now();
+		std::vector<hcm_completion> out;
+		for (auto &r : waiting) {
+			if (r->epoch > applied_done)
+				continue;
+			bool ok = true;
+			for (uint64_t key : r->targets)
+				if (world.at(key).meshed_epoch < r->epoch) {
+					ok = false;
+					break;
+				}
+			if (!ok)
+				continue;
+			hcm_completion c;
+			c.tag = r->tag;
+			c.kind = r->kind;
+			c.remeshed = (uint32_t)r->targets.size();
+			c.epoch = r->epoch;
+			c.latency_ms = std::chrono::duration<double, std::milli>(now - r->t_submit).count();
+			out.push_back(c);
+			r.reset();
+		}
+		waiting.erase(std::remove(waiting.begin(), waiting.end(), nullptr), waiting.end());
+		if (!out.empty()) {
+			std::lock_guard<std::mutex> lk(out_mu);
+			for (auto &c : out)
+				done.push_back(c);
+			completed += out.size();
+			out_cv.notify_all();
+		}
+	}
+
+	void run() {
+		for (;;) {
+			std::vector<std::unique_ptr<Request>> got;
+			{
+				std::unique_lock<std::mutex> lk(in_mu);
+				bool pending;
+				{
+					std::lock_guard<std::mutex> wl(world_mu);
+					pending = work_pending();
+				}
+				if (incoming.empty() && !pending && !stop)
+					in_cv.wait_for(lk, std::chrono::microseconds(500));
+				if (stop && incoming.empty() && !pending)
+					return;
+				while (!incoming.empty()) {
+					got.push_back(std::move(incoming.front()));
+					incoming.pop_front();
+				}
+			}
+			std::lock_guard<std::mutex> wl(world_mu);
+			for (auto &r : got)
+				apply(std::move(r));
+			for (;;) { // retire finished batches in launch order
+				BatchSlot *oldest = nullptr;
+				for (auto &b : slots)
+					if (b.busy && (!oldest || b.id < oldest->id))
+						oldest = &b;
+				if (!oldest)
+					break;
+				const hipError_t q = hipEventQuery(oldest->ev_done);
+				if (q == hipErrorNotReady)
+					break;
+				const int rc = q == hipSuccess ? finish_batch(*oldest) : HCM_ERR_HIP;
+				if (rc != HCM_OK)
+					return fail(rc);
+			}
+			complete_ready();
+			for (;;) {
+				uint32_t busy = 0;
+				BatchSlot *freeb = nullptr;
+				for (auto &b : slots) {
+					if (b.busy)
+						++busy;
+					else if (!freeb)
+						freeb = &b;
+				}
+				if (!freeb || busy >= cfg.max_inflight)
+					break;
+				const int rc = launch_batch(*freeb);
+				if (rc < 0)
+					return fail(rc);
+				if (rc > 0)
+					break;
+			}
+		}
+	}
+};
+
+static void free_server(hcm_server *s) {
+	for (auto &b : s->slots) {
+		for (void *p : {(void *)b.d_nbr, (void *)b.d_edits, (void *)b.d_ctr, (void *)b.d_res, (void *)b.d_recs})
+			if (p)
+				(void)hipFree(p);
+		for (void *p : {(void *)b.h_nbr, (void *)b.h_edits, (void *)b.h_blocks, (void *)b.h_light, (void *)b.h_res,
+		                (void *)b.h_recs, (void *)b.h_ctr})
+			if (p)
+				(void)hipHostFree(p);
+		for (hipEvent_t e : {b.ev_up, b.ev_k0, b.ev_k1, b.ev_done})
+			if (e)
+				(void)hipEventDestroy(e);
+	}
+	for (void *p : {(void *)s->d_blocks, (void *)s->d_light, (void *)s->d_verts, (void *)s->d_idx})
+		if (p)
+			(void)hipFree(p);
+	if (s->s_copy)
+		(void)hipStreamDestroy(s->s_copy);
+	if (s->s_comp)
+		(void)hipStreamDestroy(s->s_comp);
+	delete s->pool;
+	delete s;
+}
+
+template <typename T> static hipError_t dalloc(T **p, size_t count) {
+	return hipMalloc(reinterpret_cast<void **>(p), std::max<size_t>(count, 1) * sizeof(T));
+}
+template <typename T> static hipError_t halloc(T **p, size_t count) {
+	return hipHostMalloc(reinterpret_cast<void **>(p), std::max<size_t>(count, 1) * sizeof(T));
+}
+
+static int server_init(hcm_server *s) {
+	const hcm_server_config &c = s->cfg;
+	const int r = ctx_info(s->ctx, c.threads, &s->info);
+	if (r != HCM_OK)
+		return r;
+	HCM_CHECK(hipStreamCreateWithFlags(&s->s_copy, hipStreamNonBlocking));
+	HCM_CHECK(hipStreamCreateWithFlags(&s->s_comp, hipStreamNonBlocking));
+	HCM_CHECK(dalloc(&s->d_blocks, (size_t)c.max_chunks * HCM_CHUNK_VOXELS));
+	HCM_CHECK(dalloc(&s->d_light, (size_t)c.max_chunks * HCM_CHUNK_VOXELS));
+	HCM_CHECK(dalloc(&s->d_verts, (size_t)c.pool_quads * 4));
+	HCM_CHECK(dalloc(&s->d_idx, (size_t)c.pool_quads * 6));
+	s->pool = new RangeAllocator(c.pool_quads);
+	s->slots.resize(c.max_inflight);
+	for (auto &b : s->slots) {
+		HCM_CHECK(dalloc(&b.d_nbr, (size_t)c.max_batch * 27));
+		HCM_CHECK(dalloc(&b.d_edits, kMaxEdits));
+		HCM_CHECK(dalloc(&b.d_ctr, 1));
+		HCM_CHECK(dalloc(&b.d_res, c.max_batch));
+		HCM_CHECK(dalloc(&b.d_recs, s->rec_cap()));
+		HCM_CHECK(halloc(&b.h_nbr, (size_t)c.max_batch * 27));
+		HCM_CHECK(halloc(&b.h_edits, kMaxEdits));
+		HCM_CHECK(halloc(&b.h_blocks, (size_t)kMaxUploads * HCM_CHUNK_VOXELS));
+		HCM_CHECK(halloc(&b.h_light, (size_t)kMaxUploads * HCM_CHUNK_VOXELS));
+		HCM_CHECK(halloc(&b.h_res, c.max_batch));
+		HCM_CHECK(halloc(&b.h_recs, s->rec_cap()));
+		HCM_CHECK(halloc(&b.h_ctr, 1));
+		HCM_CHECK(hipEventCreateWithFlags(&b.ev_up, hipEventDisableTiming));
+		HCM_CHECK(hipEventCreate(&b.ev_k0));
+		HCM_CHECK(hipEventCreate(&b.ev_k1));
+		HCM_CHECK(hipEventCreateWithFlags(&b.ev_done, hipEventDisableTiming));
+	}
+	return HCM_OK;
+}
+
+extern "C" int hcm_server_create(hcm_ctx *ctx, const hcm_server_config *cfg, hcm_server **out) {
+	if (!out)
+		return HCM_ERR_INVALID;
+	*out = nullptr;
+	if (!ctx || !cfg || cfg->max_chunks == 0 || cfg->max_chunks > (1u << 20) || cfg->max_batch == 0 ||
+	    cfg->max_batch > (1u << 20) || cfg->max_inflight == 0 || cfg->max_inflight > 16 || cfg->pool_quads == 0 ||
+	    cfg->pool_quads >= (1ull << 32))
+		return HCM_ERR_INVALID;
+	hcm_server *s = new hcm_server;
+	s->ctx = ctx;
+	s->cfg = *cfg;
+	const int r = server_init(s);
+	if (r != HCM_OK) {
+		free_server(s);
+		return r;
+	}
+	s->thread = std::thread([s]() { s->run(); });
+	*out = s;
+	return HCM_OK;
+}
+
+extern "C" void hcm_server_destroy(hcm_server *s) {
+	if (!s)
+		return;
+	{
+		std::lock_guard<std::mutex> lk(s->in_mu);
+		s->stop = true;
+	}
+	s->in_cv.notify_all();
+	if (s->thread.joinable())
+		s->thread.join();
+	(void)hipStreamSynchronize(s->s_copy);
+	(void)hipStreamSynchronize(s->s_comp);
+	free_server(s);
+}
+
+extern "C" int hcm_server_load(hcm_server *s, int32_t cx, int32_t cy, int32_t cz, const uint16_t *blocks,
+                               const uint8_t *light, uint64_t tag) {
+	if (!s || !blocks || !light)
+		return HCM_ERR_INVALID;
+	if (cx < -(1 << 20) || cx >= (1 << 20) || cy < -(1 << 20) || cy >= (1 << 20) || cz < -(1 << 20) || cz >= (1 << 20))
+		return HCM_ERR_INVALID;
+	auto r = std::make_unique<Request>();
+	r->tag = tag;
+	r->kind = HCM_REQ_LOAD;
+	r->pos[0] = cx, r->pos[1] = cy, r->pos[2] = cz;
+	r->blocks.assign(blocks, blocks + HCM_CHUNK_VOXELS);
+	r->light.assign(light, light + HCM_CHUNK_VOXELS);
+	r->t_submit = Clock::now();
+	{
+		std::lock_guard<std::mutex> lk(s->in_mu);
+		if (s->error != HCM_OK)
+			return s->error;
+		const uint64_t key = chunk_key(cx, cy, cz);
+		if (!s->accepted.count(key)) {
+			if (s->accepted.size() >= s->cfg.max_chunks)
+				return HCM_ERR_BUSY;
+			const uint32_t slot = (uint32_t)s->accepted.size();
+			s->accepted.emplace(key, slot);
+		}
+		r->slot = s->accepted.at(key);
+		++s->submitted;
+		s->incoming.push_back(std::move(r));
+	}
+	s->in_cv.notify_one();
+	return HCM_OK;
+}
+
+extern "C" int hcm_server_edit(hcm_server *s, int32_t x, int32_t y, int32_t z, uint16_t block, uint8_t light,
+                               uint64_t tag) {
+	if (!s)
+		return HCM_ERR_INVALID;
+	auto r = std::make_unique<Request>();
+	r->tag = tag;
+	r->kind = HCM_REQ_EDIT;
+	r->pos[0] = x, r->pos[1] = y, r->pos[2] = z;
+	r->block = block;
+	r->light1 = light;
+	r->t_submit = Clock::now();
+	{
+		std::lock_guard<std::mutex> lk(s->in_mu);
+		if (s->error != HCM_OK)
+			return s->error;
+		if (!s->accepted.count(chunk_key(floor_div16(x), floor_div16(y), floor_div16(z))))
+			return HCM_ERR_INVALID;
+		++s->submitted;
+		s->incoming.push_back(std::move(r));
+	}
+	s->in_cv.notify_one();
+	return HCM_OK;
+}
+
+extern "C" int hcm_server_poll(hcm_server *s, hcm_completion *out, uint32_t cap, uint32_t *n) {
+	if (!s || !n || (cap && !out))
+		return HCM_ERR_INVALID;
+	std::lock_guard<std::mutex> lk(s->out_mu);
+	uint32_t k = 0;
+	while (k < cap && !s->done.empty()) {
+		out[k++] = s->done.front();
+		s->done.pop_front();
+	}
+	*n = k;
+	return s->error;
+}
+
+extern "C" int hcm_server_drain(hcm_server *s) {
+	if (!s)
+		return HCM_ERR_INVALID;
+	uint64_t target;
+	{
+		std::lock_guard<std::mutex> lk(s->in_mu);
+		target = s->submitted;
+	}
+	std::unique_lock<std::mutex> lk(s->out_mu);
+	while (s->completed < target && s->error == HCM_OK)
+		s->out_cv.wait_for(lk, std::chrono::milliseconds(1));
+	return s->error;
+}
+
+static int copy_meshes(const std::vector<hcm_mesh_rec> &recs, const uint2 *verts, const uint16_t *idx, bool device,
+                       hcm_host_result *out) {
+	uint64_t nv = 0, ni = 0;
+	for (auto &m : recs)
+		nv += 4ull * m.quad_count, ni += 6ull * m.quad_count;
+	out->vertex_count = (uint32_t)nv, out->index_count = (uint32_t)ni, out->mesh_count = (uint32_t)recs.size();
+	if (nv > out->vertex_capacity || ni > out->index_capacity || recs.size() > out->mesh_capacity ||
+	    (nv && (!out->vertices || !out->indices)) || (!recs.empty() && !out->meshes))
+		return HCM_ERR_CAPACITY;
+	uint32_t vb = 0, ib = 0;
+	for (size_t m = 0; m < recs.size(); ++m) {
+		const hcm_mesh_rec &rc = recs[m];
+		hcm_mesh_desc &d = out->meshes[m];
+		d.vert_begin = vb, d.vert_count = 4 * rc.quad_count, d.idx_begin = ib, d.idx_count = 6 * rc.quad_count;
+		std::memcpy(d.aabb, rc.aabb, sizeof(d.aabb));
+		d.transparent = rc.transparent;
+		const size_t vbytes = 4ull * rc.quad_count * sizeof(uint2), ibytes = 12ull * rc.quad_count;
+		if (device) {
+			HCM_CHECK(hipMemcpy(out->vertices + vb, verts + 4ull * rc.first_quad, vbytes, hipMemcpyDeviceToHost));
+			HCM_CHECK(hipMemcpy(out->indices + ib, idx + 6ull * rc.first_quad, ibytes, hipMemcpyDeviceToHost));
+		} else {
+			std::memcpy(out->vertices + vb, verts + 4ull * rc.first_quad, vbytes);
+			std::memcpy(out->indices + ib, idx + 6ull * rc.first_quad, ibytes);
+		}
+		vb += d.vert_count, ib += d.idx_count;
+	}
+	return HCM_OK;
+}
+
+extern "C" int hcm_server_chunk_mesh(hcm_server *s, int32_t cx, int32_t cy, int32_t cz, hcm_host_result *out) {
+	if (!s || !out)
+		return HCM_ERR_INVALID;
+	const int r = hcm_server_drain(s);
+	if (r != HCM_OK)
+		return r;
+	// world_mu keeps the chunk's pool range live while it is copied
+	std::lock_guard<std::mutex> wl(s->world_mu);
+	auto it = s->world.find(chunk_key(cx, cy, cz));
+	if (it == s->world.end() || !it->second.has_mesh)
+		return HCM_ERR_INVALID;
+	return copy_meshes(it->second.recs, s->d_verts, s->d_idx, true, out);
+}
+
+extern "C" int hcm_server_get_stats(hcm_server *s, hcm_server_stats *out) {
+	if (!s || !out)
+		return HCM_ERR_INVALID;
+	hcm_server_stats st;
+	{
+		std::lock_guard<std::mutex> lk(s->st_mu);
+		st = s->stats;
+	}
+	{
+		std::lock_guard<std::mutex> lk(s->in_mu);
+		st.requests = s->submitted;
+	}
+	{
+		std::lock_guard<std::mutex> lk(s->out_mu);
+		st.completed = s->completed;
+	}
+	{
+		std::lock_guard<std::mutex> lk(s->world_mu);
+		st.pool_free_quads = s->pool->free_total();
+		st.pool_used_quads = s->pool->capacity() - st.pool_free_quads;
+	}
+	*out = st;
+	return HCM_OK;
+}
+
+extern "C" int hcm_server_history_size(hcm_server *s, uint64_t *n) {
+	if (!s || !n)
+		return HCM_ERR_INVALID;
+	std::lock_guard<std::mutex> lk(s->st_mu);
+	*n = s->history.size();
+	return HCM_OK;
+}
+
+extern "C" int hcm_server_history_get(hcm_server *s, uint64_t index, int32_t pos[3], uint64_t *epoch,
+                                      hcm_host_result *out) {
+	if (!s || !pos || !epoch || !out)
+		return HCM_ERR_INVALID;
+	std::lock_guard<std::mutex> lk(s->st_mu);
+	if (index >= s->history.size())
+		return HCM_ERR_INVALID;
+	const HistoryEntry &h = s->history[index];
+	std::memcpy(pos, h.pos, sizeof(h.pos));
+	*epoch = h.epoch;
+	return copy_meshes(h.recs, h.verts.data(), h.idx.data(), false, out);
+}

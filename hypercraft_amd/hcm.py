@@ -11,7 +11,8 @@ import numpy as np
 
 from . import build as _build
 
-LIB_PATH = _build.LIB
+# HCM_LIB selects another build of the library (for example an instrumented one)
+LIB_PATH = os.environ.get("HCM_LIB", _build.LIB)
 
 HCM_OK = 0
 HCM_ERR_INVALID = -1
@@ -76,6 +77,26 @@ class TuningC(ctypes.Structure):
     _fields_ = [("wg_per_cu", ctypes.c_uint32), ("threads", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 2)]
 
 
+class ServerConfigC(ctypes.Structure):
+    _fields_ = [("max_chunks", ctypes.c_uint32), ("max_batch", ctypes.c_uint32), ("max_inflight", ctypes.c_uint32),
+                ("threads", ctypes.c_uint32), ("pool_quads", ctypes.c_uint64), ("keep_history", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 3)]
+
+
+class CompletionC(ctypes.Structure):
+    _fields_ = [("tag", ctypes.c_uint64), ("kind", ctypes.c_uint32), ("remeshed", ctypes.c_uint32),
+                ("epoch", ctypes.c_uint64), ("latency_ms", ctypes.c_double)]
+
+
+class ServerStatsC(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in ("requests", "completed", "batches", "chunks_meshed", "quads_meshed",
+                                               "retries", "pool_used_quads", "pool_free_quads")] + \
+               [("gpu_ms", ctypes.c_double)]
+
+
+REQ_LOAD = 1
+REQ_EDIT = 2
+
 JOB_RESULT_DTYPE = np.dtype([("status", "<i4"), ("first_mesh", "<u4"), ("mesh_count", "<u4"), ("quad_count", "<u4")])
 MESH_REC_DTYPE = np.dtype([("aabb", "<u4", (6,)), ("first_quad", "<u4"), ("quad_count", "<u4"), ("job", "<u4"),
                            ("transparent", "<u4")])
@@ -103,6 +124,21 @@ SIGNATURES = [
     ("hcm_batch_download", ctypes.c_int, [_VP, _VP, _VP, ctypes.c_uint32, _VP, _VP, ctypes.c_uint64]),
     ("hcm_set_tuning", ctypes.c_int, [_VP, ctypes.POINTER(TuningC)]),
     ("hcm_get_tuning", ctypes.c_int, [_VP, ctypes.POINTER(TuningC)]),
+    ("hcm_server_create", ctypes.c_int, [_VP, ctypes.POINTER(ServerConfigC), ctypes.POINTER(_VP)]),
+    ("hcm_server_destroy", None, [_VP]),
+    ("hcm_server_load", ctypes.c_int, [_VP, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _VP, _VP,
+                                       ctypes.c_uint64]),
+    ("hcm_server_edit", ctypes.c_int, [_VP, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_uint16,
+                                       ctypes.c_uint8, ctypes.c_uint64]),
+    ("hcm_server_poll", ctypes.c_int, [_VP, ctypes.POINTER(CompletionC), ctypes.c_uint32,
+                                       ctypes.POINTER(ctypes.c_uint32)]),
+    ("hcm_server_drain", ctypes.c_int, [_VP]),
+    ("hcm_server_chunk_mesh", ctypes.c_int, [_VP, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                             ctypes.POINTER(HostResultC)]),
+    ("hcm_server_get_stats", ctypes.c_int, [_VP, ctypes.POINTER(ServerStatsC)]),
+    ("hcm_server_history_size", ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_uint64)]),
+    ("hcm_server_history_get", ctypes.c_int, [_VP, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int32),
+                                              ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(HostResultC)]),
     ("hcm_gen_terrain_chunk", None, [ctypes.c_uint64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                      ctypes.c_uint32, _VP, _VP]),
     ("hcm_gen_cave_chunk", None, [ctypes.c_uint64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
@@ -312,6 +348,77 @@ class Batch:
     def close(self):
         if self._ptr:
             lib().hcm_batch_destroy(self._ptr)
+            self._ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous batching of chunk loads and voxel edits (hcm_server)."""
+
+    def __init__(self, ctx, max_chunks, max_batch=4096, max_inflight=2, threads=0, pool_quads=1 << 24,
+                 keep_history=False):
+        self.ctx = ctx
+        cfg = ServerConfigC(max_chunks, max_batch, max_inflight, threads, pool_quads, int(keep_history),
+                            (ctypes.c_uint32 * 3)(0, 0, 0))
+        self._ptr = ctypes.c_void_p()
+        check("hcm_server_create", lib().hcm_server_create(ctx.ptr, ctypes.byref(cfg), ctypes.byref(self._ptr)))
+        self._out = _HostOut()
+        self._done = (CompletionC * 4096)()
+
+    def load(self, cx, cy, cz, blocks, light, tag=0):
+        b = np.ascontiguousarray(blocks, np.uint16).reshape(4096)
+        l = np.ascontiguousarray(light, np.uint8).reshape(4096)
+        check("hcm_server_load", lib().hcm_server_load(self._ptr, cx, cy, cz, b.ctypes.data, l.ctypes.data, tag))
+
+    def edit(self, x, y, z, block, light, tag=0):
+        check("hcm_server_edit", lib().hcm_server_edit(self._ptr, x, y, z, block, light, tag))
+
+    def poll(self):
+        """Completed requests: list of dicts (tag, kind, remeshed, epoch, latency_ms)."""
+        out = []
+        while True:
+            n = ctypes.c_uint32()
+            check("hcm_server_poll", lib().hcm_server_poll(self._ptr, self._done, len(self._done), ctypes.byref(n)))
+            for k in range(n.value):
+                c = self._done[k]
+                out.append({"tag": c.tag, "kind": c.kind, "remeshed": c.remeshed, "epoch": c.epoch,
+                            "latency_ms": c.latency_ms})
+            if n.value < len(self._done):
+                return out
+
+    def drain(self):
+        check("hcm_server_drain", lib().hcm_server_drain(self._ptr))
+
+    def chunk_mesh(self, cx, cy, cz):
+        check("hcm_server_chunk_mesh", lib().hcm_server_chunk_mesh(self._ptr, cx, cy, cz, ctypes.byref(self._out.c)))
+        return self._out.meshes()
+
+    def stats(self):
+        st = ServerStatsC()
+        check("hcm_server_get_stats", lib().hcm_server_get_stats(self._ptr, ctypes.byref(st)))
+        return {n: getattr(st, n) for n, _ in ServerStatsC._fields_}
+
+    def history(self):
+        """Every (chunk position, epoch, meshes) result recorded with keep_history."""
+        n = ctypes.c_uint64()
+        check("hcm_server_history_size", lib().hcm_server_history_size(self._ptr, ctypes.byref(n)))
+        out = []
+        pos = (ctypes.c_int32 * 3)()
+        ep = ctypes.c_uint64()
+        for i in range(n.value):
+            check("hcm_server_history_get", lib().hcm_server_history_get(self._ptr, i, pos, ctypes.byref(ep),
+                                                                          ctypes.byref(self._out.c)))
+            out.append((tuple(pos), ep.value, self._out.meshes()))
+        return out
+
+    def close(self):
+        if self._ptr:
+            lib().hcm_server_destroy(self._ptr)
             self._ptr = ctypes.c_void_p()
 
     def __del__(self):

@@ -15,10 +15,10 @@ from . import hcm
 _REV = (1, -1, 0)
 SLOT_OFFSETS = np.array([(_REV[s // 9], _REV[(s // 3) % 3], _REV[s % 3]) for s in range(27)], np.int32)
 
-CONFIG1_SEED = 1
-CONFIG2_SEED = 2
-CONFIG3_SEED = 3
-CONFIG4_SEED = 4
+TERRAIN_SEED = 1
+CAVE_SEED = 2
+RANDOM_SEED = 3
+MIXED_SEED = 4
 
 
 def _gen_many(fn, seed, coords, flags=0, threads=8):
@@ -56,7 +56,7 @@ def _grid_coords(lo, dims):
     return g + np.asarray(lo, np.int64)
 
 
-def config1(seed=CONFIG1_SEED, flags=0):
+def terrain_chunk(seed=TERRAIN_SEED, flags=0):
     """One 16^3 terrain chunk with its 26 neighbours: the 48^3 voxels x, z in [-16, 32), y in [0, 48),
     whose heightmap surface (y in [8, 40]) crosses the centre chunk (chunk (0, 1, 0))."""
     coords = _grid_coords((-1, 0, -1), (3, 3, 3))
@@ -65,7 +65,7 @@ def config1(seed=CONFIG1_SEED, flags=0):
     return blocks, light, nbr
 
 
-def terrain_region(n=(16, 16, 16), seed=CONFIG1_SEED, flags=0, y0=-5):
+def terrain_region(n=(16, 16, 16), seed=TERRAIN_SEED, flags=0, y0=-5):
     """A region of terrain chunks (n chunks each way, lowest chunk y = y0) plus a one-chunk rim."""
     dims = (n[0] + 2, n[1] + 2, n[2] + 2)
     coords = _grid_coords((-1, y0 - 1, -1), dims)
@@ -74,7 +74,7 @@ def terrain_region(n=(16, 16, 16), seed=CONFIG1_SEED, flags=0, y0=-5):
     return blocks, light, region_jobs(dims, centers)
 
 
-def config2(seed=CONFIG2_SEED, n=16, flags=0):
+def cave_region(seed=CAVE_SEED, n=16, flags=0):
     """Cave terrain: a n^3-chunk region (chunks 0..n-1) plus a one-chunk rim, one job per region chunk."""
     dims = (n + 2, n + 2, n + 2)
     coords = _grid_coords((-1, -1, -1), dims)
@@ -83,7 +83,7 @@ def config2(seed=CONFIG2_SEED, n=16, flags=0):
     return blocks, light, region_jobs(dims, centers)
 
 
-def config3(seed=CONFIG3_SEED, n_jobs=1024):
+def random_dense(seed=RANDOM_SEED, n_jobs=1024):
     """Worst-case density: every job has its own 27 i.i.d. random chunks."""
     blocks = np.zeros((27 * n_jobs, 4096), np.uint16)
     light = np.zeros((27 * n_jobs, 4096), np.uint8)

@@ -197,6 +197,67 @@ typedef struct hcm_tuning {
 int hcm_set_tuning(hcm_ctx *ctx, const hcm_tuning *t);
 int hcm_get_tuning(hcm_ctx *ctx, hcm_tuning *t);
 
+/* ------------------------------------------------------------------------------------------------- */
+/* Serving: continuous batching of chunk-load and block-edit requests over device-resident chunks.     */
+/* No counterpart in the reference, whose worker threads pop one mesh task per chunk                  */
+/* (client/src/ChunkTaskPool.cpp:87-130) after ChunkTaskData<kMesh>::Pop finds all 27 chunks present  */
+/* (client/src/ChunkMeshTask.cpp:14-39); an edit there pushes remesh tasks for nearby chunks          */
+/* (client/src/ChunkSetBlockTask.cpp:100-119).                                                        */
+/* ------------------------------------------------------------------------------------------------- */
+typedef struct hcm_server hcm_server;
+
+typedef struct hcm_server_config {
+	uint32_t max_chunks;   /* device chunk slots */
+	uint32_t max_batch;    /* chunks meshed per launch */
+	uint32_t max_inflight; /* batches queued on the device at once */
+	uint32_t threads;      /* workgroup size of the meshing kernel (0 = context tuning) */
+	uint64_t pool_quads;   /* vertex pool for the meshes kept on the device */
+	uint32_t keep_history; /* 1: keep a host copy of every batch's meshes (tests) */
+	uint32_t reserved[3];
+} hcm_server_config;
+
+#define HCM_REQ_LOAD 1u
+#define HCM_REQ_EDIT 2u
+
+typedef struct hcm_completion {
+	uint64_t tag;
+	uint32_t kind;       /* HCM_REQ_LOAD or HCM_REQ_EDIT */
+	uint32_t remeshed;   /* chunks this request waited on */
+	uint64_t epoch;      /* requests applied to the world before the last batch it waited on */
+	double latency_ms;   /* submit to completion, host clock */
+} hcm_completion;
+
+typedef struct hcm_server_stats {
+	uint64_t requests, completed, batches, chunks_meshed, quads_meshed, retries;
+	uint64_t pool_used_quads, pool_free_quads;
+	double gpu_ms; /* sum of the meshing batches' HIP-event times */
+} hcm_server_stats;
+
+int hcm_server_create(hcm_ctx *ctx, const hcm_server_config *cfg, hcm_server **out);
+void hcm_server_destroy(hcm_server *srv);
+/* Loads chunk (cx, cy, cz): copies 4096 blocks + 4096 light bytes to a device slot; the chunk and every
+ * loaded neighbour whose 26 neighbours are then all present are (re)meshed. HCM_ERR_BUSY when no slot. */
+int hcm_server_load(hcm_server *srv, int32_t cx, int32_t cy, int32_t cz, const uint16_t *blocks,
+                    const uint8_t *light, uint64_t tag);
+/* Sets the block and light byte of world voxel (x, y, z) in a loaded chunk and remeshes the chunks whose
+ * 18^3 halo holds the voxel (the edited chunk and 0..7 neighbours). HCM_ERR_INVALID if not loaded. */
+int hcm_server_edit(hcm_server *srv, int32_t x, int32_t y, int32_t z, uint16_t block, uint8_t light, uint64_t tag);
+/* Completed requests since the last poll (up to cap); *n receives the count. */
+int hcm_server_poll(hcm_server *srv, hcm_completion *out, uint32_t cap, uint32_t *n);
+/* Blocks until every submitted request has completed. Returns the server's sticky error, if any
+ * (HCM_ERR_POOL_FULL when the live meshes leave no room in the vertex pool, HCM_ERR_HIP). */
+int hcm_server_drain(hcm_server *srv);
+/* Drains, then copies the current meshes of a chunk to host buffers (same layout as hcm_mesh_chunk);
+ * HCM_ERR_INVALID if the chunk has no mesh. Requests submitted meanwhile from other threads may replace
+ * the meshes while they are copied: call it when no other thread submits. */
+int hcm_server_chunk_mesh(hcm_server *srv, int32_t cx, int32_t cy, int32_t cz, hcm_host_result *out);
+int hcm_server_get_stats(hcm_server *srv, hcm_server_stats *out);
+
+/* With keep_history: number of (batch, chunk) meshing results recorded, and one of them with its epoch
+ * (requests applied to the world before its batch) and meshes. */
+int hcm_server_history_size(hcm_server *srv, uint64_t *n);
+int hcm_server_history_get(hcm_server *srv, uint64_t index, int32_t pos[3], uint64_t *epoch, hcm_host_result *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -31,10 +31,10 @@ from hypercraft_amd import workloads  # noqa: E402
 def golden_sets():
     """name -> (blocks, light, nbr). Shared with the tests, which rebuild the same inputs."""
     sets = {
-        "config1": workloads.config1(),
-        "config1_plants": workloads.config1(flags=1),
-        "config2": workloads.config2(),
-        "config3": workloads.config3(),
+        "terrain_chunk": workloads.terrain_chunk(),
+        "terrain_chunk_plants": workloads.terrain_chunk(flags=1),
+        "cave_region": workloads.cave_region(),
+        "random_dense": workloads.random_dense(),
         "leaf_checkerboard": cases.leaf_checkerboard(),
         "lone_stone": cases.lone_block(1),
         "lone_glass": cases.lone_block(9, sun=9),
@@ -63,11 +63,11 @@ def main():
             meshes = ref.mesh(b27, l27)
             dig[j] = np.frombuffer(meshdigest.digest(meshes), np.uint8)
             quads[j] = meshdigest.quads(meshes)
-            if name == "config1":
+            if name == "terrain_chunk":
                 for k, m in enumerate(meshes):
-                    out[f"config1_mesh{k}_vertices"] = m.vertices
-                    out[f"config1_mesh{k}_indices"] = m.indices
-                    out[f"config1_mesh{k}_meta"] = np.array([*m.aabb, m.transparent], np.uint32)
+                    out[f"terrain_chunk_mesh{k}_vertices"] = m.vertices
+                    out[f"terrain_chunk_mesh{k}_indices"] = m.indices
+                    out[f"terrain_chunk_mesh{k}_meta"] = np.array([*m.aabb, m.transparent], np.uint32)
         out[name + "_digest"] = dig
         out[name + "_quads"] = quads
         out[name + "_input"] = np.frombuffer(meshdigest.input_digest(blocks, light, nbr).encode(), np.uint8)

@@ -44,7 +44,7 @@ def test_device_present():
 
 
 @pytest.mark.parametrize("name", ["lone_stone", "lone_glass", "slab", "uniform_air", "uniform_stone",
-                                  "uniform_water", "uniform_flow", "config1", "config1_plants",
+                                  "uniform_water", "uniform_flow", "terrain_chunk", "terrain_chunk_plants",
                                   "leaf_checkerboard"])
 def test_plain_path_single_chunks(ctx, oracle, golden, name):
     blocks, light, nbr = gset(name)
@@ -56,13 +56,13 @@ def test_plain_path_single_chunks(ctx, oracle, golden, name):
     assert meshdigest.digest(got) == meshdigest.digest(want)
 
 
-def test_plain_path_config1_full_meshes(ctx, golden):
-    blocks, light, nbr = gset("config1")
+def test_plain_path_terrain_chunk_full_meshes(ctx, golden):
+    blocks, light, nbr = gset("terrain_chunk")
     got = ctx.mesh_chunk(*workloads.slice_jobs(blocks, light, nbr, 0))
     for k, m in enumerate(got):
-        assert np.array_equal(m.vertices, golden[f"config1_mesh{k}_vertices"])
-        assert np.array_equal(m.indices, golden[f"config1_mesh{k}_indices"])
-        assert list(m.aabb) + [m.transparent] == golden[f"config1_mesh{k}_meta"].tolist()
+        assert np.array_equal(m.vertices, golden[f"terrain_chunk_mesh{k}_vertices"])
+        assert np.array_equal(m.indices, golden[f"terrain_chunk_mesh{k}_indices"])
+        assert list(m.aabb) + [m.transparent] == golden[f"terrain_chunk_mesh{k}_meta"].tolist()
 
 
 @pytest.mark.parametrize("fam", list(cases.FAMILIES))
@@ -101,7 +101,7 @@ def assert_digests(name, got, want):
     assert bad.size == 0, f"{name}: {bad.size} jobs differ, first {bad[:10].tolist()}"
 
 
-@pytest.mark.parametrize("name", ["palette_" + f for f in cases.FAMILIES] + ["config3", "config2"])
+@pytest.mark.parametrize("name", ["palette_" + f for f in cases.FAMILIES] + ["random_dense", "cave_region"])
 def test_batched_persistent(ctx, golden, name):
     blocks, light, nbr = gset(name)
     say(f"batched persistent: {name} ({nbr.shape[0]} jobs)")
@@ -112,11 +112,11 @@ def test_batched_persistent(ctx, golden, name):
 
 
 def test_per_chunk_launches_match(ctx, golden):
-    blocks, light, nbr = gset("config2")
+    blocks, light, nbr = gset("cave_region")
     sub = nbr[::37]
-    say(f"per-chunk launches: config2 subset ({sub.shape[0]} jobs)")
+    say(f"per-chunk launches: cave_region subset ({sub.shape[0]} jobs)")
     out = run_batch(ctx, blocks, light, sub, mode=hcm.RUN_PER_CHUNK)
-    assert_digests("config2 subset", digests_of(out), golden["config2_digest"][::37])
+    assert_digests("cave_region subset", digests_of(out), golden["cave_region_digest"][::37])
 
 
 @pytest.mark.parametrize("threads", [128, 512])

@@ -1,0 +1,266 @@
+"""Serving (hcm_server): continuous batching of chunk loads and voxel edits (needs an MI355X).
+
+The claim under test is that every mesh the server produces in the mix of concurrent batches is the
+mesh of that chunk meshed alone, by the CPU oracle, on the world as it stood at the mesh's epoch
+(the number of requests applied before its batch) -- bit-exact -- and that every request completes
+once, after every chunk it made stale holds a mesh at or after its epoch.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import cases
+import meshdigest
+from hypercraft_amd import hcm, workloads
+
+pytestmark = pytest.mark.gpu
+
+SEED = 11
+DIMS = (4, 3, 4)  # chunk grid x, y, z: the inner 2 x 1 x 2 chunks have all 26 neighbours
+EDIT_PALETTE = np.concatenate([cases.CUSTOM, cases.GLASSY, cases.FLOW, cases.LOGS, [0, 0, 0, 1, 2, 3]]).astype(np.uint16)
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+class World:
+    """Host copy of the world: chunk position -> (blocks, light), edited like the server's."""
+
+    def __init__(self):
+        self.chunks = {}
+
+    def load(self, pos, blocks, light):
+        self.chunks[pos] = (blocks.copy(), light.copy())
+
+    def edit(self, x, y, z, block, light):
+        c = (x // 16, y // 16, z // 16)
+        i = (x % 16) + 16 * ((z % 16) + 16 * (y % 16))
+        b, l = self.chunks[c]
+        b[i], l[i] = block, light
+
+    def meshable(self, pos):
+        return all(tuple(int(v) for v in np.add(pos, d)) in self.chunks for d in workloads.SLOT_OFFSETS)
+
+    def mesh_alone(self, oracle, pos):
+        nb = [self.chunks[tuple(int(v) for v in np.add(pos, d))] for d in workloads.SLOT_OFFSETS]
+        return oracle.mesh([b for b, _ in nb], [l for _, l in nb])
+
+
+def terrain(pos):
+    return hcm.gen_terrain_chunk(SEED, *pos)
+
+
+def grid():
+    return [(x, y, z) for y in range(DIMS[1]) for z in range(DIMS[2]) for x in range(DIMS[0])]
+
+
+def inner():
+    return [p for p in grid() if all(0 < p[a] < DIMS[a] - 1 for a in range(3))]
+
+
+def check_history(srv, oracle, log):
+    """Replays the request log and compares every recorded mesh with the oracle alone at its epoch."""
+    hist = sorted(srv.history(), key=lambda h: h[1])
+    world = World()
+    applied = 0
+    for pos, epoch, meshes in hist:
+        while applied < epoch:
+            op = log[applied]
+            if op[0] == "load":
+                world.load(*op[1:])
+            else:
+                world.edit(*op[1:])
+            applied += 1
+        assert world.meshable(pos), (pos, epoch)
+        want = world.mesh_alone(oracle, pos)
+        assert meshdigest.digest(meshes) == meshdigest.digest(want), f"chunk {pos} at epoch {epoch}"
+    return len(hist)
+
+
+def check_final(srv, oracle, world):
+    for pos in world.chunks:
+        if world.meshable(pos):
+            got = srv.chunk_mesh(*pos)
+            assert meshdigest.digest(got) == meshdigest.digest(world.mesh_alone(oracle, pos)), pos
+
+
+def check_completions(done, n_requests, log):
+    tags = sorted(c["tag"] for c in done)
+    assert tags == list(range(n_requests)), "every request completes exactly once"
+    for c in done:
+        assert c["epoch"] == c["tag"] + 1  # one submitting thread: request k has epoch k + 1
+        assert c["kind"] == (hcm.REQ_LOAD if log[c["tag"]][0] == "load" else hcm.REQ_EDIT)
+        assert c["latency_ms"] >= 0
+
+
+def load_all(srv, world, log, order):
+    for pos in order:
+        b, l = terrain(pos)
+        srv.load(*pos, b, l, tag=len(log))
+        world.load(pos, b, l)
+        log.append(("load", pos, b, l))
+
+
+def test_server_loads_match_meshing_alone(ctx, oracle):
+    rng = np.random.default_rng(1)
+    order = [grid()[i] for i in rng.permutation(len(grid()))]
+    srv = hcm.Server(ctx, max_chunks=64, max_batch=3, max_inflight=3, pool_quads=1 << 22, keep_history=True)
+    try:
+        world, log = World(), []
+        say("serve: loading 48 chunks in random order")
+        load_all(srv, world, log, order)
+        srv.drain()
+        done = srv.poll()
+        check_completions(done, len(log), log)
+        n = check_history(srv, oracle, log)
+        assert n >= len(inner())
+        check_final(srv, oracle, world)
+        st = srv.stats()
+        assert st["completed"] == st["requests"] == len(log)
+        assert st["chunks_meshed"] == n
+    finally:
+        srv.close()
+
+
+def halo_chunks(world, x, y, z):
+    """Meshable chunks whose 18^3 halo holds world voxel (x, y, z)."""
+    out = set()
+    for dx in (-1, 0, 1):
+        for dy in (-1, 0, 1):
+            for dz in (-1, 0, 1):
+                c = ((x + dx) // 16, (y + dy) // 16, (z + dz) // 16)
+                if world.meshable(c):
+                    out.add(c)
+    return out
+
+
+def _edits(rng, n):
+    """Random voxel edits in the inner chunks, half of them on chunk borders (halo of the neighbours)."""
+    out = []
+    for _ in range(n):
+        c = inner()[rng.integers(len(inner()))]
+        v = rng.integers(0, 16, 3)
+        if rng.random() < 0.5:
+            a = rng.integers(3)
+            v[a] = 0 if rng.random() < 0.5 else 15
+        if rng.random() < 0.2:
+            v[:] = [0 if rng.random() < 0.5 else 15 for _ in range(3)]  # corner: seven neighbours
+        x, y, z = (int(16 * c[a] + v[a]) for a in range(3))
+        out.append((x, y, z, int(EDIT_PALETTE[rng.integers(len(EDIT_PALETTE))]), int(rng.integers(256))))
+    return out
+
+
+@pytest.mark.parametrize("max_batch,max_inflight", [(1, 1), (4, 3), (64, 2)])
+def test_server_edits_match_meshing_alone(ctx, oracle, max_batch, max_inflight):
+    rng = np.random.default_rng(100 + max_batch)
+    srv = hcm.Server(ctx, max_chunks=64, max_batch=max_batch, max_inflight=max_inflight, pool_quads=1 << 22,
+                     keep_history=True)
+    try:
+        world, log = World(), []
+        load_all(srv, world, log, grid())
+        say(f"serve: 300 edits, max_batch {max_batch}, max_inflight {max_inflight}")
+        for k, (x, y, z, blk, lt) in enumerate(_edits(rng, 300)):
+            srv.edit(x, y, z, blk, lt, tag=len(log))
+            world.edit(x, y, z, blk, lt)
+            log.append(("edit", x, y, z, blk, lt))
+            if k % 50 == 49:
+                srv.drain()
+        srv.drain()
+        done = srv.poll()
+        check_completions(done, len(log), log)
+        for c in done:
+            if c["kind"] == hcm.REQ_EDIT:
+                _, x, y, z = log[c["tag"]][:4]
+                assert c["remeshed"] == len(halo_chunks(world, x, y, z))
+        n = check_history(srv, oracle, log)
+        say(f"serve: {n} recorded meshes equal the oracle alone at their epochs")
+        check_final(srv, oracle, world)
+    finally:
+        srv.close()
+
+
+def test_server_reload_replaces_chunk(ctx, oracle):
+    srv = hcm.Server(ctx, max_chunks=64, max_batch=8, max_inflight=2, pool_quads=1 << 22, keep_history=True)
+    try:
+        world, log = World(), []
+        load_all(srv, world, log, grid())
+        say("serve: reload the inner chunks with cave chunks, edits in between")
+        rng = np.random.default_rng(5)
+        for pos in inner():
+            b, l = hcm.gen_cave_chunk(SEED, *pos)
+            srv.load(*pos, b, l, tag=len(log))
+            world.load(pos, b, l)
+            log.append(("load", pos, b, l))
+            for x, y, z, blk, lt in _edits(rng, 5):
+                srv.edit(x, y, z, blk, lt, tag=len(log))
+                world.edit(x, y, z, blk, lt)
+                log.append(("edit", x, y, z, blk, lt))
+        srv.drain()
+        check_completions(srv.poll(), len(log), log)
+        check_history(srv, oracle, log)
+        check_final(srv, oracle, world)
+    finally:
+        srv.close()
+
+
+def test_server_pool_pressure(ctx, oracle):
+    """A pool with room for two arenas: batches wait for memory and still produce the same meshes."""
+    srv = hcm.Server(ctx, max_chunks=64, max_batch=2, max_inflight=4, pool_quads=3 << 16, keep_history=True)
+    try:
+        world, log = World(), []
+        load_all(srv, world, log, grid())
+        rng = np.random.default_rng(9)
+        say("serve: edits under pool pressure")
+        for x, y, z, blk, lt in _edits(rng, 100):
+            srv.edit(x, y, z, blk, lt, tag=len(log))
+            world.edit(x, y, z, blk, lt)
+            log.append(("edit", x, y, z, blk, lt))
+        srv.drain()
+        check_completions(srv.poll(), len(log), log)
+        check_history(srv, oracle, log)
+        check_final(srv, oracle, world)
+        st = srv.stats()
+        assert st["pool_used_quads"] + st["pool_free_quads"] == 3 << 16
+        live = sum(sum(len(m.vertices) // 4 for m in srv.chunk_mesh(*p)) for p in inner())
+        assert st["pool_used_quads"] == live  # superseded meshes and unused arena tails went back to the pool
+    finally:
+        srv.close()
+
+
+def test_server_pool_full_is_reported(ctx):
+    srv = hcm.Server(ctx, max_chunks=64, max_batch=4, max_inflight=2, pool_quads=64)
+    try:
+        world, log = World(), []
+        say("serve: pool of 64 quads")
+        with pytest.raises(hcm.HcmError) as e:
+            load_all(srv, world, log, grid())
+            srv.drain()
+        assert e.value.code == hcm.HCM_ERR_POOL_FULL
+    finally:
+        srv.close()
+
+
+def test_server_rejects_bad_requests(ctx):
+    srv = hcm.Server(ctx, max_chunks=2, max_batch=4, max_inflight=1, pool_quads=1 << 20)
+    try:
+        b, l = terrain((0, 0, 0))
+        with pytest.raises(hcm.HcmError) as e:
+            srv.edit(1, 2, 3, 1, 0)
+        assert e.value.code == hcm.HCM_ERR_INVALID  # chunk not loaded
+        srv.load(0, 0, 0, b, l)
+        srv.load(-1, 0, 0, b, l)
+        srv.load(0, 0, 0, b, l)  # a reload takes no new slot
+        with pytest.raises(hcm.HcmError) as e:
+            srv.load(5, 0, 0, b, l)
+        assert e.value.code == hcm.HCM_ERR_BUSY
+        srv.edit(-1, 0, 0, 1, 0)  # voxel (-1, 0, 0) lies in chunk (-1, 0, 0)
+        srv.drain()
+        assert len(srv.poll()) == 4
+        with pytest.raises(hcm.HcmError) as e:
+            srv.chunk_mesh(0, 0, 0)  # not meshable: neighbours missing
+        assert e.value.code == hcm.HCM_ERR_INVALID
+    finally:
+        srv.close()

@@ -35,16 +35,16 @@ def test_generators_are_deterministic_and_match_golden_inputs():
     from hypercraft_amd import workloads
 
     golden = np.load(os.path.join(ROOT, "tests", "golden", "golden.npz"))
-    b, l, n = workloads.config1()
-    assert meshdigest.input_digest(b, l, n).encode() == golden["config1_input"].tobytes()
-    b, l, n = workloads.config3(n_jobs=1024)
-    assert meshdigest.input_digest(b, l, n).encode() == golden["config3_input"].tobytes()
+    b, l, n = workloads.terrain_chunk()
+    assert meshdigest.input_digest(b, l, n).encode() == golden["terrain_chunk_input"].tobytes()
+    b, l, n = workloads.random_dense(n_jobs=1024)
+    assert meshdigest.input_digest(b, l, n).encode() == golden["random_dense_input"].tobytes()
 
 
-def test_config3_mix():
+def test_random_dense_mix():
     from hypercraft_amd import hcm, workloads
 
-    b, l, n = workloads.config3(n_jobs=16)
+    b, l, n = workloads.random_dense(n_jobs=16)
     opaque = set(hcm.opaque_values().tolist())
     assert len(opaque) == 33
     v = b.ravel()

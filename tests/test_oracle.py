@@ -41,7 +41,7 @@ def sets():
     return SETS
 
 
-SET_NAMES = ["config1", "config1_plants", "config2", "config3", "leaf_checkerboard", "lone_stone", "lone_glass",
+SET_NAMES = ["terrain_chunk", "terrain_chunk_plants", "cave_region", "random_dense", "leaf_checkerboard", "lone_stone", "lone_glass",
              "slab", "uniform_air", "uniform_stone", "uniform_water", "uniform_flow"] + \
             [p + f for f in cases.FAMILIES for p in ("palette_", "palette_sparse_")]
 
