@@ -1,13 +1,21 @@
-"""Benchmark: chunks meshed per second on one MI355X (batched persistent path vs per-chunk launches).
+"""Benchmark: chunks meshed per second on one MI355X (batched persistent path vs per-chunk launches),
+plus the serving loop's p50/p99 remesh latency.
 
-Workload (the configuration the metric is quoted on): the 4096-chunk cave-terrain batch -- a
-16x16x16-chunk region (256^3 voxels) plus a one-chunk rim, seeded 3D fBm caves under a surface
-cap, ~15% all-air and ~10% all-stone chunks, spatially correlated random sun/torch light. Inputs
-are resident in HBM before timing. One step = one batched meshing pass over all 4096 chunks
-(HIP-graph replay of counter reset + persistent kernel). Output is checked bit-exact against
-the digests the reference mesher produced for the same chunks.
+Headline workload: the 4096-chunk cave-terrain batch -- a 16x16x16-chunk region (256^3 voxels) plus
+a one-chunk rim, seeded 3D fBm caves under a surface cap, ~15% all-air and ~10% all-stone chunks,
+spatially correlated random sun/torch light. Inputs are resident in HBM before timing. One step =
+one batched meshing pass over all 4096 chunks (HIP-graph replay of counter reset + persistent
+kernel) with the autotuned workgroup shape. Output is checked bit-exact against the digests the
+reference mesher produced for the same chunks.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1]
+Two more legs go into the same JSON line:
+  mixed_batches  200 batches of log-uniform size in [1, 4096] drawn from the cave region and a
+                 plant-covered terrain region: captured-graph runs vs per-chunk launches, bit-exact;
+  serving        a spiral load trace (radius 24 columns x 16 chunks, ~29k terrain chunks, Poisson
+                 arrivals at 20k chunks/s) with concurrent single-voxel edits at 2k/s through the
+                 continuous-batching server (16 GB vertex pool): p50/p99 request latency.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-plain] [--no-mixed] [--no-serve]
 """
 import argparse
 import json
@@ -30,8 +38,9 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(blocks, light, nbr, threads):
-    """The reference mesher (oracle/_ref, if it was built) else the C oracle, on the host cores."""
+def cpu_baseline(blocks, light, nbr, threads, min_seconds=10.0):
+    """The reference mesher (oracle/_ref, if it was built) else the C oracle, on the host cores:
+    whole passes over the batch until at least min_seconds of wall time have run."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import cpu_oracle
 
@@ -39,12 +48,117 @@ def cpu_baseline(blocks, light, nbr, threads):
         impl, kind = cpu_oracle.Reference(), "reference"
     else:
         impl, kind = cpu_oracle.Oracle(os.path.join(ROOT, "tests", "golden", "registry.bin")), "port"
+    passes, quads = 0, 0
     t0 = time.perf_counter()
-    q = impl.mesh_many(blocks, light, nbr, threads)
-    dt = time.perf_counter() - t0
-    return {"value": round(nbr.shape[0] / dt, 1), "unit": "chunks/s", "cores": threads, "kind": kind,
-            "sample": f"all {nbr.shape[0]} chunks of the batch once ({int(q.sum())} quads), {dt:.2f} s wall",
-            "quads_per_s": round(float(q.sum()) / dt, 1)}
+    while True:
+        quads += int(impl.mesh_many(blocks, light, nbr, threads).sum())
+        passes += 1
+        dt = time.perf_counter() - t0
+        if dt >= min_seconds:
+            break
+    chunks = passes * nbr.shape[0]
+    return {"value": round(chunks / dt, 1), "unit": "chunks/s", "cores": threads, "kind": kind,
+            "sample": f"{passes} passes over the {nbr.shape[0]}-chunk batch ({chunks} chunks, {quads} quads), "
+                      f"{dt:.2f} s wall, {threads} threads",
+            "quads_per_s": round(quads / dt, 1)}
+
+
+def percentile(a, q):
+    return round(float(np.percentile(np.asarray(a, np.float64), q)), 4) if len(a) else None
+
+
+def mixed_leg(ctx, hcm, workloads, meshdigest, golden):
+    """Uneven batches: captured-graph run (cold: capture + first replay; warm: replay) vs per-chunk launches."""
+    blocks, light, nbr, _ = workloads.mixed_pool()
+    want = np.concatenate([golden["cave_region_digest"], golden["terrain_plants_region_digest"]])
+    store = hcm.Store(ctx, blocks.shape[0])
+    store.upload(blocks, light)
+    batches = workloads.mixed_batches(nbr.shape[0])
+    chunks = graph_ms = plain_ms = cold_s = 0.0
+    bad = 0
+    for k, jobs in enumerate(batches):
+        b = hcm.Batch(ctx, store, nbr[jobs])
+        t0 = time.perf_counter()
+        b.run()
+        b.sync()
+        cold_s += time.perf_counter() - t0
+        b.run()
+        graph_ms += b.sync()
+        out = b.download()
+        for i, j in enumerate(jobs):
+            bad += meshdigest.digest(hcm.job_meshes(i, *out)) != want[j].tobytes()
+        b.run(hcm.RUN_PER_CHUNK)
+        plain_ms += b.sync()
+        chunks += jobs.size
+        b.close()
+        if k % 50 == 49:
+            log(f"mixed: {k + 1} batches")
+    store.close()
+    return {"batches": len(batches), "chunks": int(chunks), "bit_exact": bad == 0,
+            "graph_chunks_per_s": round(chunks / (graph_ms * 1e-3), 1),
+            "graph_cold_chunks_per_s": round(chunks / cold_s, 1),
+            "per_chunk_launch_chunks_per_s": round(chunks / (plain_ms * 1e-3), 1),
+            "graph_vs_per_chunk": round(plain_ms / graph_ms, 2)}
+
+
+def serve_leg(ctx, hcm, workloads):
+    """Replays the spiral load + edit trace against the server in real time; latency per request kind."""
+    log("serving: generating the spiral trace")
+    tr = workloads.spiral_trace()
+    coords, blocks, light, events = tr["coords"], tr["blocks"], tr["light"], tr["events"]
+    pool_quads = (16 << 30) // 44  # a 16 GB vertex pool (44 B per quad)
+    srv = hcm.Server(ctx, max_chunks=coords.shape[0], max_batch=4096, max_inflight=2, pool_quads=pool_quads)
+    kinds = {}
+    lag = 0.0
+    log(f"serving: {len(events)} requests over {events[-1][0]:.2f} s")
+    t0 = time.perf_counter()
+    for tag, (t, kind, payload) in enumerate(events):
+        now = time.perf_counter() - t0
+        while now < t:
+            now = time.perf_counter() - t0
+        lag = max(lag, now - t)
+        if kind == "load":
+            c = coords[payload]
+            srv.load(int(c[0]), int(c[1]), int(c[2]), blocks[payload], light[payload], tag)
+        else:
+            srv.edit(*payload, tag)
+        kinds[tag] = kind
+    srv.drain()
+    wall = time.perf_counter() - t0
+    done = srv.poll()
+    st = srv.stats()
+    srv.close()
+    lat = {"load": [], "edit": []}
+    for c in done:
+        lat[kinds[c["tag"]]].append(c["latency_ms"])
+    res = {"requests": len(events), "completed": len(done), "loads": len(lat["load"]), "edits": len(lat["edit"]),
+           "wall_s": round(wall, 3), "submit_lag_max_ms": round(lag * 1e3, 3),
+           "chunks_meshed": int(st["chunks_meshed"]), "quads_meshed": int(st["quads_meshed"]),
+           "chunks_meshed_per_s": round(st["chunks_meshed"] / wall, 1), "batches": int(st["batches"]),
+           "gpu_busy_frac": round(st["gpu_ms"] * 1e-3 / wall, 4), "retries": int(st["retries"])}
+    for k in ("edit", "load"):
+        res[f"{k}_p50_ms"] = percentile(lat[k], 50)
+        res[f"{k}_p99_ms"] = percentile(lat[k], 99)
+        res[f"{k}_max_ms"] = round(max(lat[k]), 4) if lat[k] else None
+    return res
+
+
+def copy_calibration(nbytes, reps):
+    """Device-to-device hipMemcpy of nbytes, reps times: the known-size copy that scales the HBM
+    counters (FETCH_SIZE / WRITE_SIZE) read for the meshing kernel."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(src), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(dst), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(src, 1, ctypes.c_size_t(nbytes)) == 0
+    for _ in range(reps):
+        assert hip.hipMemcpy(dst, src, ctypes.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print(json.dumps({"copy_bytes": nbytes, "reps": reps}), flush=True)
 
 
 def main():
@@ -54,12 +168,20 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run a device-to-device copy of BYTES (for counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration, 5)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build", file=sys.stderr)
         sys.exit(2)
 
-    from hypercraft_amd import hcm, workloads
+    from hypercraft_amd import hcm, tune, workloads
 
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import meshdigest
@@ -68,11 +190,17 @@ def main():
     log("generating the 4096-chunk cave batch")
     blocks, light, nbr = workloads.cave_region()
     golden = np.load(os.path.join(ROOT, "tests", "golden", "golden.npz"))
-    ctx = hcm.Context(pool_quads=1 << 23)
+    ctx = hcm.Context(pool_quads=1 << 24)
     store = hcm.Store(ctx, blocks.shape[0])
     store.upload(blocks, light)
     batch = hcm.Batch(ctx, store, nbr)
     n = nbr.shape[0]
+
+    tuning = ctx.get_tuning()
+    if not args.no_tune:
+        log("autotune")
+        tuned = tune.autotune(ctx, batch, workload="cave_region", reps=3, log=log)
+        tuning = {"threads": tuned["threads"], "wg_per_cu": tuned["wg_per_cu"]}
 
     log(f"warmup {args.warmup}")
     for _ in range(args.warmup):
@@ -125,7 +253,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "cave-terrain batch: 4096 chunks of 16^3 (16x16x16-chunk region, 256^3 voxels "
                                "+ rim), seed 2, one persistent launch per step",
-                   "chunks": n, "quads_per_step": int(quads), "meshes_per_step": int(meshes)},
+                   "chunks": n, "quads_per_step": int(quads), "meshes_per_step": int(meshes),
+                   "threads": tuning["threads"], "wg_per_cu": tuning["wg_per_cu"]},
         "quads_per_s": round(quads * args.steps / dt, 1),
         "bit_exact": bool(bit_exact),
         "plain_per_chunk_chunks_per_s": round(plain, 1) if plain else None,
@@ -134,6 +263,14 @@ def main():
                      "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
                      "kernel_ms": round(k_ms, 4), "algorithmic_bytes": int(alg_bytes)},
     }
+    batch.close()
+    store.close()
+    if not args.no_mixed:
+        log("mixed uneven batches")
+        res["mixed_batches"] = mixed_leg(ctx, hcm, workloads, meshdigest, golden)
+    if not args.no_serve:
+        log("serving trace")
+        res["serving"] = serve_leg(ctx, hcm, workloads)
     if not args.no_cpu:
         log("CPU baseline")
         res["cpu_baseline"] = cpu_baseline(blocks, light, nbr, min(16, os.cpu_count() or 1))

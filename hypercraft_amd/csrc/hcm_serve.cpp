@@ -171,6 +171,7 @@ struct BatchSlot { // device and pinned buffers of one in-flight batch
 	bool busy = false;
 	bool arena_was_largest = false; // the arena took the largest free range there was
 	uint64_t id = 0, epoch = 0;     // epoch: every request up to it is applied on the device
+	uint32_t recs_copied = 0;       // mesh records copied back with the results (the usual count)
 	std::vector<uint64_t> jobs;
 	Range arena;
 };
@@ -432,7 +433,9 @@ struct hcm_server {
 			HCM_CHECK(hipMemcpyAsync(b.h_res, b.d_res, b.jobs.size() * sizeof(hcm_job_result), hipMemcpyDeviceToHost,
 			                         s_comp));
 			HCM_CHECK(hipMemcpyAsync(b.h_ctr, b.d_ctr, sizeof(Counters), hipMemcpyDeviceToHost, s_comp));
-			HCM_CHECK(hipMemcpyAsync(b.h_recs, b.d_recs, rec_cap() * sizeof(hcm_mesh_rec), hipMemcpyDeviceToHost, s_comp));
+			b.recs_copied = std::min<uint32_t>(rec_cap(), 3 * (uint32_t)b.jobs.size() + 64);
+			HCM_CHECK(hipMemcpyAsync(b.h_recs, b.d_recs, b.recs_copied * sizeof(hcm_mesh_rec), hipMemcpyDeviceToHost,
+			                         s_comp));
 		}
 		HCM_CHECK(hipEventRecord(b.ev_done, s_comp));
 		return 0;
@@ -444,6 +447,10 @@ struct hcm_server {
 		if (!b.jobs.empty()) {
 			float ms = 0;
 			HCM_CHECK(hipEventElapsedTime(&ms, b.ev_k0, b.ev_k1));
+			const uint32_t n_recs = std::min(b.h_ctr->mesh_cursor, rec_cap());
+			if (n_recs > b.recs_copied) // more overflow meshes than usual: fetch the rest
+				HCM_CHECK(hipMemcpy(b.h_recs + b.recs_copied, b.d_recs + b.recs_copied,
+				                    (n_recs - b.recs_copied) * sizeof(hcm_mesh_rec), hipMemcpyDeviceToHost));
 			std::vector<uint2> hv;
 			std::vector<uint16_t> hi;
 			const uint64_t used = std::min<uint64_t>(b.h_ctr->pool_cursor, b.arena.count);

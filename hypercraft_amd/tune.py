@@ -19,7 +19,9 @@ WG_PER_CU = (0, 1, 2, 3, 4)  # 0: as many as the LDS footprint allows
 def _output_digest(batch):
     """Digest of every job's mesh list in job order (pool positions vary from run to run)."""
     res, recs, verts, idx = batch.download()
-    h = hashlib.sha256(res[["status", "mesh_count", "quad_count"]].tobytes())
+    h = hashlib.sha256()
+    for f in ("status", "mesh_count", "quad_count"):
+        h.update(np.ascontiguousarray(res[f]).tobytes())
     for r in res:
         for m in recs[r["first_mesh"]:r["first_mesh"] + r["mesh_count"]]:
             q0, n = int(m["first_quad"]), int(m["quad_count"])

@@ -19,6 +19,7 @@ TERRAIN_SEED = 1
 CAVE_SEED = 2
 RANDOM_SEED = 3
 MIXED_SEED = 4
+TRACE_SEED = 5
 
 
 def _gen_many(fn, seed, coords, flags=0, threads=8):
@@ -99,6 +100,89 @@ def random_dense(seed=RANDOM_SEED, n_jobs=1024):
         list(ex.map(work, [(i, min(n, i + step)) for i in range(0, n, step)]))
     nbr = np.arange(27 * n_jobs, dtype=np.uint32).reshape(n_jobs, 27)
     return blocks, light, nbr
+
+
+def terrain_plants_region(seed=MIXED_SEED):
+    """Heightmap terrain with grass plants (cross meshes) on 5% of grass tops: a 16 x 3 x 16-chunk
+    region (chunk y 0..2 holds the surface band, y in [8, 40]) plus a one-chunk rim; 768 jobs."""
+    return terrain_region(n=(16, 3, 16), seed=seed, flags=hcm.HCM_GEN_PLANTS, y0=0)
+
+
+def mixed_pool():
+    """One store holding the cave region and the plant-terrain region. Returns (blocks, light, nbr,
+    n_cave): jobs [0, n_cave) are the cave region's, the rest the plant terrain's."""
+    cb, cl, cn = cave_region()
+    tb, tl, tn = terrain_plants_region()
+    blocks = np.concatenate([cb, tb])
+    light = np.concatenate([cl, tl])
+    nbr = np.concatenate([cn, tn + np.uint32(cb.shape[0])])
+    return blocks, light, nbr, cn.shape[0]
+
+
+def mixed_batches(n_pool, seed=MIXED_SEED, n_batches=200, max_size=4096):
+    """Uneven batches: sizes log-uniform in [1, max_size], jobs drawn without replacement from a pool."""
+    rng = np.random.default_rng(seed)
+    sizes = np.floor(np.exp(rng.uniform(0.0, np.log(max_size + 1), n_batches))).astype(np.int64)
+    sizes = np.clip(sizes, 1, min(max_size, n_pool))
+    return [np.sort(rng.choice(n_pool, size=int(k), replace=False)).astype(np.int64) for k in sizes]
+
+
+def spiral_columns(radius):
+    """Chunk columns (x, z) with x^2 + z^2 <= radius^2, nearest first, counter-clockwise within a ring."""
+    r = int(radius)
+    xs, zs = np.meshgrid(np.arange(-r, r + 1), np.arange(-r, r + 1), indexing="ij")
+    xs, zs = xs.ravel(), zs.ravel()
+    keep = xs * xs + zs * zs <= r * r
+    xs, zs = xs[keep], zs[keep]
+    order = np.lexsort((np.arctan2(zs, xs), xs * xs + zs * zs))
+    return np.stack([xs[order], zs[order]], 1)
+
+
+def spiral_trace(radius=24, height=16, y0=-7, load_rate=20000.0, edit_rate=2000.0, seed=TRACE_SEED):
+    """A serving trace: terrain chunks loaded column by column in spiral order (Poisson arrivals at
+    load_rate per second) while single-voxel edits arrive (Poisson, edit_rate per second) at random
+    voxels of chunks whose 26 neighbours have already arrived.
+
+    Returns dict(coords (N, 3), blocks (N, 4096), light (N, 4096), events: list of
+    (time_s, kind, payload)) with kind "load" -> chunk index, "edit" -> (x, y, z, block, light)."""
+    cols = spiral_columns(radius)
+    ys = np.arange(y0, y0 + height)
+    coords = np.array([(x, y, z) for x, z in cols for y in ys], np.int64)
+    blocks, light = _gen_many("hcm_gen_terrain_chunk", seed, coords)
+    n = coords.shape[0]
+    rng = np.random.default_rng(seed)
+    t_load = np.cumsum(rng.exponential(1.0 / load_rate, n))
+    # arrival index of each chunk's last neighbour: it becomes meshable when that one has arrived
+    index = {tuple(c): i for i, c in enumerate(coords.tolist())}
+    ready = np.full(n, -1, np.int64)
+    for i, c in enumerate(coords.tolist()):
+        last = 0
+        for d in SLOT_OFFSETS.tolist():
+            j = index.get((c[0] + d[0], c[1] + d[1], c[2] + d[2]))
+            if j is None:
+                last = -1
+                break
+            last = max(last, j)
+        ready[i] = last
+    order = np.argsort(np.where(ready < 0, n + 1, ready), kind="stable")
+    ready_sorted = np.where(ready < 0, n + 1, ready)[order]
+    palette = np.array([0, 0, 0, 1, 1, 2, 3, 4, 9, 0x10, 0x210, 14, 5, 0x105, 19, 20, 23, 0x121], np.uint16)
+    events = [(float(t), "load", i) for i, t in enumerate(t_load)]
+    t, t_end = 0.0, float(t_load[-1])
+    while True:
+        t += rng.exponential(1.0 / edit_rate)
+        if t >= t_end:
+            break
+        arrived = int(np.searchsorted(t_load, t))  # loads submitted before t
+        k = int(np.searchsorted(ready_sorted, arrived - 1, side="right"))
+        if k == 0:
+            continue
+        c = coords[order[rng.integers(k)]]
+        v = rng.integers(0, 16, 3)
+        events.append((t, "edit", (int(16 * c[0] + v[0]), int(16 * c[1] + v[1]), int(16 * c[2] + v[2]),
+                                   int(palette[rng.integers(len(palette))]), int(rng.integers(256)))))
+    events.sort(key=lambda e: e[0])
+    return {"coords": coords, "blocks": blocks, "light": light, "events": events}
 
 
 def slice_jobs(blocks, light, nbr, jobs):

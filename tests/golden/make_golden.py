@@ -9,7 +9,7 @@ compiled from its own sources). The fixtures are data only:
                 quad count and the digest of its exact mesh list (tests/meshdigest.py);
                 for the single terrain chunk also the full reference meshes.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [set ...]   (no names: every set)
 """
 import os
 import sys
@@ -35,6 +35,7 @@ def golden_sets():
         "terrain_chunk_plants": workloads.terrain_chunk(flags=1),
         "cave_region": workloads.cave_region(),
         "random_dense": workloads.random_dense(),
+        "terrain_plants_region": workloads.terrain_plants_region(),
         "leaf_checkerboard": cases.leaf_checkerboard(),
         "lone_stone": cases.lone_block(1),
         "lone_glass": cases.lone_block(9, sun=9),
@@ -50,11 +51,18 @@ def golden_sets():
     return sets
 
 
-def main():
+def main(only=None):
+    """Writes every set, or with names given only those (merged into the existing golden.npz)."""
     ref = cpu_oracle.Reference()
-    ref.export_registry(os.path.join(HERE, "registry.bin"))
+    path = os.path.join(HERE, "golden.npz")
     out = {}
+    if only:
+        out.update(np.load(path))
+    else:
+        ref.export_registry(os.path.join(HERE, "registry.bin"))
     for name, (blocks, light, nbr) in golden_sets().items():
+        if only and name not in only:
+            continue
         n = nbr.shape[0]
         dig = np.zeros((n, 32), np.uint8)
         quads = np.zeros(n, np.uint32)
@@ -72,8 +80,8 @@ def main():
         out[name + "_quads"] = quads
         out[name + "_input"] = np.frombuffer(meshdigest.input_digest(blocks, light, nbr).encode(), np.uint8)
         print(f"{name}: {n} jobs, {int(quads.sum())} quads")
-    np.savez_compressed(os.path.join(HERE, "golden.npz"), **out)
+    np.savez_compressed(path, **out)
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

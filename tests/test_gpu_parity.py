@@ -101,7 +101,8 @@ def assert_digests(name, got, want):
     assert bad.size == 0, f"{name}: {bad.size} jobs differ, first {bad[:10].tolist()}"
 
 
-@pytest.mark.parametrize("name", ["palette_" + f for f in cases.FAMILIES] + ["random_dense", "cave_region"])
+@pytest.mark.parametrize("name", ["palette_" + f for f in cases.FAMILIES] +
+                         ["random_dense", "cave_region", "terrain_plants_region"])
 def test_batched_persistent(ctx, golden, name):
     blocks, light, nbr = gset(name)
     say(f"batched persistent: {name} ({nbr.shape[0]} jobs)")
@@ -109,6 +110,26 @@ def test_batched_persistent(ctx, golden, name):
     assert (out[0]["status"] == 0).all()
     assert np.array_equal(out[0]["quad_count"], golden[name + "_quads"])
     assert_digests(name, digests_of(out), golden[name + "_digest"])
+
+
+def test_mixed_uneven_batches(ctx, golden):
+    """Uneven batches drawn from one store holding the cave region and the plant terrain: each batch
+    is a captured graph run once and replayed, and per-chunk launches give the same meshes."""
+    blocks, light, nbr, n_cave = workloads.mixed_pool()
+    want = np.concatenate([golden["cave_region_digest"], golden["terrain_plants_region_digest"]])
+    store = hcm.Store(ctx, blocks.shape[0])
+    store.upload(blocks, light)
+    try:
+        for k, jobs in enumerate(workloads.mixed_batches(nbr.shape[0], n_batches=12)):
+            say(f"mixed batch {k}: {jobs.size} jobs")
+            batch = hcm.Batch(ctx, store, nbr[jobs])
+            for mode in (hcm.RUN_PERSISTENT, hcm.RUN_PERSISTENT, hcm.RUN_PER_CHUNK):
+                batch.run(mode)
+                batch.sync()
+                assert_digests(f"mixed batch {k} mode {mode}", digests_of(batch.download()), want[jobs])
+            batch.close()
+    finally:
+        store.close()
 
 
 def test_per_chunk_launches_match(ctx, golden):

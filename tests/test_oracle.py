@@ -41,8 +41,9 @@ def sets():
     return SETS
 
 
-SET_NAMES = ["terrain_chunk", "terrain_chunk_plants", "cave_region", "random_dense", "leaf_checkerboard", "lone_stone", "lone_glass",
-             "slab", "uniform_air", "uniform_stone", "uniform_water", "uniform_flow"] + \
+SET_NAMES = ["terrain_chunk", "terrain_chunk_plants", "cave_region", "random_dense", "terrain_plants_region",
+             "leaf_checkerboard", "lone_stone", "lone_glass", "slab", "uniform_air", "uniform_stone", "uniform_water",
+             "uniform_flow"] + \
             [p + f for f in cases.FAMILIES for p in ("palette_", "palette_sparse_")]
 
 
