@@ -63,6 +63,16 @@ def cpu_baseline(blocks, light, nbr, threads, min_seconds=10.0):
             "quads_per_s": round(quads / dt, 1)}
 
 
+def measured_traffic():
+    """HBM bytes per persistent launch of the cave batch, from the committed rocprofv3 counter summary."""
+    path = os.path.join(ROOT, "profiles", "mesh_kernel_cave_region.md")
+    if os.path.exists(path):
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1])
+    return None
+
+
 def percentile(a, q):
     return round(float(np.percentile(np.asarray(a, np.float64), q)), 4) if len(a) else None
 
@@ -260,7 +270,8 @@ def main():
         "plain_per_chunk_chunks_per_s": round(plain, 1) if plain else None,
         "batched_vs_plain": round(value / plain, 2) if plain else None,
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 2), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK, 4), "traffic": measured_traffic(),
+                     "traffic_source": "profiles/mesh_kernel_cave_region.md (FETCH_SIZE + WRITE_SIZE, copy-scaled)",
                      "kernel_ms": round(k_ms, 4), "algorithmic_bytes": int(alg_bytes)},
     }
     batch.close()
