@@ -12,6 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT, "libhcm.so")
+WRAPPER_CHECK = os.path.join(OUT, "hcm_wrapper_check")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
@@ -54,6 +55,11 @@ def build(verbose=False):
     _run([[HIPCC, "--offload-arch=" + ARCH, "-shared", "-o", tmp] + [obj(u) for u in hip_units + cxx_units]],
          verbose)
     os.replace(tmp, LIB)
+    # the C++ front end (include/hcm.hpp) built into a small check program the tests run
+    root = os.path.dirname(HERE)
+    _run([["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(root, "include"),
+           os.path.join(root, "tests", "cpp", "wrapper_check.cpp"), "-o", WRAPPER_CHECK,
+           "-L", OUT, "-lhcm", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib"]], verbose)
     return LIB
 
 

@@ -8,16 +8,15 @@
 //      bitmaps mark cells with custom meshes and liquid-flow cells. A chunk whose halo holds a single
 //      block value and no custom mesh has no faces and stops here.
 //   2. liquid-flow cells take their per-cell face textures (the reference's m_dynamic_block_textures).
-//   3. visibility: for every (axis, slice, direction) task a wavefront classifies 64 cells per round
-//      (BlockTexture::Show on the class bits; texture codes only for transparent against transparent)
-//      and writes the 16x16 visible mask by ballot.
-//   4. light words: the visible faces are numbered densely (task-major, row-major, as the reference
-//      scans them), light4_init is evaluated once per visible face, and the equal-to-right,
-//      equal-to-below and transparent-pass masks are set from neighbouring faces' words.
-//   5. greedy merge: one lane per task replays the reference's merge row by row on the bit masks:
+//   3. one wavefront per (axis, slice, direction) task walks the 16x16 mask in groups of 4 rows:
+//      BlockTexture::Show on the class bits gives the visible faces (texture codes are compared only
+//      for transparent against transparent), light4_init gives each visible face's word, and lane
+//      shuffles compare words with the right and lower neighbours; ballots write the visible,
+//      equal-to-right, equal-to-below and transparent-pass masks.
+//   4. greedy merge: one lane per task replays the reference's merge row by row on the bit masks:
 //      the quads starting in a row are the row's uncovered visible runs split where the face word
 //      changes; each extends downward while every column of it continues (equal and uncovered).
-//   6. block-wide prefix sums give every quad its index in its pass and its position in the
+//   5. block-wide prefix sums give every quad its index in its pass and its position in the
 //      reference's emission order; one atomic per chunk allocates its output range; quads are then
 //      described into an LDS list and written one lane per quad as packed BlockVertex words and u16
 //      indices, with the reference's 65,532-vertex mesh split, per-mesh AABBs and mesh order.
@@ -32,7 +31,6 @@ constexpr int kHalo = 18;
 constexpr int kHaloVox = kHalo * kHalo * kHalo;
 constexpr int kHaloRows = kHalo * kHalo;
 constexpr int kTasks = 102; // 3 axes x 17 slices x 2 directions
-constexpr int kFaceCap = 1024;
 constexpr int kListCap = 1536;
 constexpr uint32_t kSplitQuads = 16383u; // a BlockMesh holds at most 65,532 vertices (:273-279, :397-403)
 constexpr int kMaxMeshPerPass = 8;
@@ -102,18 +100,16 @@ __device__ __forceinline__ uint32_t local16(int x, int y, int z) {
 }
 __device__ __forceinline__ int hidx(int x, int y, int z) { return (x + 1) * kSX + (y + 1) * kSY + (z + 1) * kSZ; }
 
-struct FaceStage { // dense visible faces of a group of tasks
-	uint32_t desc[kFaceCap]; // task | u << 7 | v << 11
-	uint2 key[kFaceCap];     // Light4 words; key.y top byte = texture code
-};
-
 template <int NT> struct Smem {
 	uint32_t vox[kHaloVox];
 	// per task and mask row: visible, equal-to-right, equal-to-below, transparent pass
-	uint16_t ne[kTasks][16], eh[kTasks][16], ev[kTasks][16], tp[kTasks][16];
+	alignas(16) uint16_t ne[kTasks][16];
+	alignas(16) uint16_t eh[kTasks][16];
+	alignas(16) uint16_t ev[kTasks][16];
+	alignas(16) uint16_t tp[kTasks][16];
 	union {
-		FaceStage fs;
-		uint2 list[kListCap];
+		uint2 key[kListCap];  // face words of the current group of tasks, in dense order
+		uint2 list[kListCap]; // quad descriptors of the current emission window
 	};
 	uint32_t tpre[kTasks + 2]; // prefix of visible faces per task
 	uint2 rows[kMaxPid];
@@ -184,17 +180,21 @@ __device__ __forceinline__ uint2 light4(const uint32_t *vox, int face, int self)
 	const uint32_t mm = vox[f - sp - sq], mp = vox[f - sp + sq], pmq = vox[f + sp - sq], ppq = vox[f + sp + sq];
 	const bool wa = face == 0 || face == 3 || face == 4;
 	const uint32_t pass3 = (F >> 8) & 1u, fs = F & 15u, ft = (F >> 4) & 15u;
-	uint32_t ao[4], sun[4], torch[4];
-	light_vertex(qm, mm, pm, pass3, fs, ft, ao[0], sun[0], torch[0]);  // (P, Q) signs (-,-)
-	light_vertex(qp, ppq, pp, pass3, fs, ft, ao[2], sun[2], torch[2]); // (+,+)
-	light_vertex(wa ? qm : qp, wa ? pmq : mp, wa ? pp : pm, pass3, fs, ft, ao[1], sun[1], torch[1]);
-	light_vertex(wa ? qp : qm, wa ? mp : pmq, wa ? pm : pp, pass3, fs, ft, ao[3], sun[3], torch[3]);
 	uint32_t lo = 0, hi = 0, ao4 = 0;
-#pragma unroll
+	// one vertex at a time (small register footprint): (P, Q) signs (-,-), then (+,-) or (-,+) by the
+	// winding, then (+,+), then the other of (+,-) and (-,+)
+#pragma unroll 1
 	for (int k = 0; k < 4; ++k) {
-		lo |= sun[k] << (6 * k);
-		hi |= torch[k] << (6 * k);
-		ao4 |= ao[k] << (2 * k);
+		const bool pplus = k == 2 || (k == 1 && wa) || (k == 3 && !wa);
+		const bool qplus = k == 2 || (k == 1 && !wa) || (k == 3 && wa);
+		const uint32_t c0 = qplus ? qp : qm;                                  // side cell along Q
+		const uint32_t c1 = pplus ? (qplus ? ppq : pmq) : (qplus ? mp : mm); // corner cell
+		const uint32_t c2 = pplus ? pp : pm;                                  // side cell along P
+		uint32_t ao, sun, torch;
+		light_vertex(c0, c1, c2, pass3, fs, ft, ao, sun, torch);
+		lo |= sun << (6 * k);
+		hi |= torch << (6 * k);
+		ao4 |= ao << (2 * k);
 	}
 	return make_uint2(lo | (ao4 << 24), hi);
 }
@@ -424,11 +424,7 @@ __device__ __forceinline__ hcm_mesh_face custom_face(const MeshParams &p, const 
 // ------------------------------------------------------------------------------------------------
 // writing one quad
 // ------------------------------------------------------------------------------------------------
-__device__ __forceinline__ void put_quad(const MeshParams &p, uint32_t slot, const uint32_t w0[4], const uint32_t w1[4],
-                                         uint32_t base, bool flip) {
-	uint4 *vp = reinterpret_cast<uint4 *>(p.pool_verts + 4 * (size_t)slot);
-	vp[0] = make_uint4(w0[0], w1[0], w0[1], w1[1]);
-	vp[1] = make_uint4(w0[2], w1[2], w0[3], w1[3]);
+__device__ __forceinline__ void put_indices(const MeshParams &p, uint32_t slot, uint32_t base, bool flip) {
 	// index patterns: flip (0,1,2, 0,2,3) (:440-446) -- also the custom-mesh pattern (:295-301) --
 	// and no flip (1,2,3, 0,1,3) (:453-459)
 	const uint32_t b0 = base, b1 = base + 1, b2 = base + 2, b3 = base + 3;
@@ -442,6 +438,14 @@ __device__ __forceinline__ void put_quad(const MeshParams &p, uint32_t slot, con
 		ip[1] = b3 | (b0 << 16);
 		ip[2] = b1 | (b3 << 16);
 	}
+}
+
+__device__ __forceinline__ void put_quad(const MeshParams &p, uint32_t slot, const uint32_t w0[4], const uint32_t w1[4],
+                                         uint32_t base, bool flip) {
+	uint4 *vp = reinterpret_cast<uint4 *>(p.pool_verts + 4 * (size_t)slot);
+	vp[0] = make_uint4(w0[0], w1[0], w0[1], w1[1]);
+	vp[1] = make_uint4(w0[2], w1[2], w0[3], w1[3]);
+	put_indices(p, slot, base, flip);
 }
 
 // BlockVertex word 1 (client/include/client/BlockVertex.hpp:13-14)
@@ -461,18 +465,11 @@ __device__ __forceinline__ void aabb_merge(Smem<NT> &s, int pass, int k, const u
 	atomicMax(&a[5], hi[2]);
 }
 
-// Mesh split bookkeeping for quad i of a pass: the vertex base its indices use (the reference reads the
-// 16-bit vertex count before splitting, so the first quad of a continuation mesh keeps base 65,532).
-__device__ __forceinline__ uint32_t index_base(uint32_t i) {
-	const uint32_t k = i / kSplitQuads, j = i % kSplitQuads;
-	return (j == 0 && k > 0) ? 65532u : 4u * j;
-}
-
 // Regular quad: task t = 34 * axis + 2 * slice + direction, origin (u, v0) on the mask, width w along V,
 // height h along U (:357-460).
 template <int NT>
 __device__ __forceinline__ void emit_regular(const MeshParams &p, Smem<NT> &s, uint32_t desc, uint32_t slot,
-                                          uint32_t pass, uint32_t i) {
+                                          uint32_t vbase, uint32_t lo[3], uint32_t hi[3]) {
 	const int t = desc & 127, u = (desc >> 7) & 15, v0 = (desc >> 11) & 15;
 	const int w = int((desc >> 15) & 15) + 1, h = int((desc >> 19) & 15) + 1;
 	const int A = t / 34, sl = (t % 34) >> 1, inv = t & 1, face = 2 * A + inv;
@@ -502,14 +499,12 @@ __device__ __forceinline__ void emit_regular(const MeshParams &p, Smem<NT> &s, u
 		w0[k] = x | (y << 10) | (z << 20) | (uint32_t(A) << 30);
 		w1[k] = vword1(tex, uint32_t(face), l4_ao(l, k), l4_sun(l, k), l4_torch(l, k));
 	}
-	put_quad(p, slot, w0, w1, index_base(i), l4_flip(l));
-	uint32_t lo[3], hi[3];
+	put_quad(p, slot, w0, w1, vbase, l4_flip(l));
 #pragma unroll
 	for (int a = 0; a < 3; ++a) {
 		lo[a] = uint32_t(o[a]) << 4;
 		hi[a] = uint32_t(o[a] + du[a] + dv[a]) << 4;
 	}
-	aabb_merge(s, pass, i / kSplitQuads, lo, hi);
 }
 
 __device__ __forceinline__ uint32_t lerp16(uint32_t a, uint32_t b, uint32_t t) { return a * (16u - t) + b * t; }
@@ -523,8 +518,8 @@ __device__ __forceinline__ uint32_t interp(uint32_t l0, uint32_t l1, uint32_t l2
 }
 
 template <int NT>
-__device__ __forceinline__ void emit_custom(const MeshParams &p, Smem<NT> &s, uint32_t desc, uint32_t slot, uint32_t pass,
-                                         uint32_t i) {
+__device__ __forceinline__ void emit_custom(const MeshParams &p, Smem<NT> &s, uint32_t desc, uint32_t slot,
+                                         uint32_t vbase, uint32_t lo[3], uint32_t hi[3]) {
 	const int cell = desc & 4095, j = (desc >> 12) & 15;
 	const int x = cell & 15, z = (cell >> 4) & 15, y = cell >> 8;
 	const CustomCell c = custom_cell(p, s, cell);
@@ -540,10 +535,18 @@ __device__ __forceinline__ void emit_custom(const MeshParams &p, Smem<NT> &s, ui
 		ua = va, va = tmp;
 	}
 	const uint32_t bx = uint32_t(x) << 4, by = uint32_t(y) << 4, bz = uint32_t(z) << 4;
-	uint32_t w0[4], w1[4], lo[3] = {~0u, ~0u, ~0u}, hi[3] = {0, 0, 0};
+	lo[0] = lo[1] = lo[2] = ~0u;
+	hi[0] = hi[1] = hi[2] = 0u;
+	// vertices one at a time (custom faces are rare; this keeps the register footprint small), each
+	// picked from packed words so that nothing is indexed at run time
+	uint32_t vw[4];
 #pragma unroll
+	for (int k = 0; k < 4; ++k)
+		vw[k] = uint32_t(f.v[k].x) | uint32_t(f.v[k].y) << 8 | uint32_t(f.v[k].z) << 16 | uint32_t(f.v[k].ao) << 24;
+#pragma unroll 1
 	for (int k = 0; k < 4; ++k) {
-		const uint32_t vx = f.v[k].x, vy = f.v[k].y, vz = f.v[k].z;
+		const uint32_t w = k == 0 ? vw[0] : (k == 1 ? vw[1] : (k == 2 ? vw[2] : vw[3]));
+		const uint32_t vx = w & 0xffu, vy = (w >> 8) & 0xffu, vz = (w >> 16) & 0xffu;
 		auto pick = [&](int a) { return a == 0 ? vx : (a == 1 ? vy : vz); };
 		const uint32_t px = bx + vx, py = by + vy, pz = bz + vz;
 		lo[0] = min(lo[0], px), lo[1] = min(lo[1], py), lo[2] = min(lo[2], pz);
@@ -552,7 +555,7 @@ __device__ __forceinline__ void emit_custom(const MeshParams &p, Smem<NT> &s, ui
 		uint32_t dw = pick(la) < 16u ? pick(la) : 16u;
 		if (lf & 1)
 			dw = 16u - dw;
-		uint32_t ao = f.v[k].ao;
+		uint32_t ao = w >> 24;
 		if (ao == 4)
 			ao = interp(l4_ao(lo4, 0), l4_ao(lo4, 1), l4_ao(lo4, 2), l4_ao(lo4, 3), l4_ao(hi4, 0), l4_ao(hi4, 1),
 			            l4_ao(hi4, 2), l4_ao(hi4, 3), du, dv, dw) & 0xffu;
@@ -561,82 +564,166 @@ __device__ __forceinline__ void emit_custom(const MeshParams &p, Smem<NT> &s, ui
 		const uint32_t torch = interp(l4_torch(lo4, 0), l4_torch(lo4, 1), l4_torch(lo4, 2), l4_torch(lo4, 3),
 		                              l4_torch(hi4, 0), l4_torch(hi4, 1), l4_torch(hi4, 2), l4_torch(hi4, 3), du, dv,
 		                              dw) & 0xffu;
-		w0[k] = px | (py << 10) | (pz << 20) | (uint32_t(f.axis) << 30);
-		w1[k] = vword1(f.texture, f.render_face, ao, sun, torch);
+		p.pool_verts[4 * (size_t)slot + k] = make_uint2(px | (py << 10) | (pz << 20) | (uint32_t(f.axis) << 30),
+		                                                vword1(f.texture, f.render_face, ao, sun, torch));
 	}
-	put_quad(p, slot, w0, w1, index_base(i), true);
-	aabb_merge(s, pass, i / kSplitQuads, lo, hi);
+	put_indices(p, slot, vbase, true);
 }
 
 // ------------------------------------------------------------------------------------------------
-// visibility of one (axis, slice, direction) task: one wavefront, 4 rounds of 4 mask rows
+// visibility by bit slices: one lane per (u, v) column along the slicing axis
 // ------------------------------------------------------------------------------------------------
-template <int NT> __device__ __forceinline__ uint32_t task_visibility(Smem<NT> &s, int t) {
+// The lane reads the column's 18 halo voxels and keeps the 2-bit face classes of its + and - faces as
+// bit planes over the column (bit a = halo position a). The face between halo positions a and a + 1 is
+// in slice a; BlockTexture::Show (texture/include/texture/BlockTexture.hpp:68-76) on classes is
+//   opaque self:      visible unless the other face is opaque,
+//   transparent self: visible if the other is empty or liquid, or transparent with another texture,
+//   liquid self:      visible if the other is empty,
+// and only blocks inside the chunk own faces (:330, :335). Ballots over the wave transpose the columns'
+// bits into the 16x16 masks of the (axis, slice, direction) tasks, four mask rows per wave.
+template <int NT> __device__ __forceinline__ void visibility_column(Smem<NT> &s, int A, int col) {
 	const int lane = threadIdx.x & 63;
-	const int A = t / 34, rem = t % 34, sl = rem >> 1, inv = rem & 1, face = 2 * A + inv;
-	uint32_t n = 0;
-	if ((sl == 0 && !inv) || (sl == 16 && inv)) { // no block owns these faces (:330, :335)
-		if (lane < 16)
-			s.ne[t][lane] = 0;
-		return 0;
-	}
-	const int sa = axis_stride(A), su = axis_stride(u_axis(A)), sv = axis_stride(v_axis(A));
-	const int layer = inv ? sl : sl - 1; // the block owning the face
-	const int r = lane >> 4, v = lane & 15;
-	const int other_off = inv ? -sa : sa;
+	const int u = col >> 4, v = col & 15;
+	const int sa = axis_stride(A);
+	const int base = (u + 1) * axis_stride(u_axis(A)) + (v + 1) * axis_stride(v_axis(A));
+	const int shp = kVoxClassShift + 4 * A; // class of face 2A (+); face 2A + 1 (-) follows
+	uint32_t p0 = 0, p1 = 0, m0 = 0, m1 = 0;
 #pragma unroll
-	for (int q = 0; q < 4; ++q) {
-		const int self = (layer + 1) * sa + (4 * q + r + 1) * su + (v + 1) * sv;
-		const uint32_t vs = s.vox[self], vo = s.vox[self + other_off];
-		const uint32_t ca = face_class(vs, face), cb = face_class(vo, face ^ 1);
-		// BlockTexture::Show (texture/include/texture/BlockTexture.hpp:68-76) on face classes
-		bool vis = (ca == kClassOpaque && cb != kClassOpaque) || (ca == kClassLiquid && cb == kClassEmpty);
-		if (ca == kClassTransparent && cb != kClassOpaque)
-			vis = cb != kClassTransparent ||
-			      row_code(s, vs >> kVoxPidShift, face) != row_code(s, vo >> kVoxPidShift, face ^ 1);
-		const uint64_t b = ballot64(vis);
-		if (lane == 0)
-			*reinterpret_cast<uint64_t *>(&s.ne[t][4 * q]) = b;
-		n += __popcll(b);
+	for (int a = 0; a < kHalo; ++a) {
+		const uint32_t w = s.vox[base + a * sa] >> shp;
+		p0 |= (w & 1u) << a;
+		p1 |= ((w >> 1) & 1u) << a;
+		m0 |= ((w >> 2) & 1u) << a;
+		m1 |= ((w >> 3) & 1u) << a;
 	}
-	return n;
-}
-
-// Light words of one visible face (dense index i of the current group); sets its transparent-pass bit.
-template <int NT> __device__ __forceinline__ void face_key(Smem<NT> &s, int i) {
-	const uint32_t d = s.fs.desc[i];
-	const int t = d & 127, u = (d >> 7) & 15, v = (d >> 11) & 15;
-	const int A = t / 34, sl = (t % 34) >> 1, inv = t & 1, face = 2 * A + inv;
-	const int layer = inv ? sl : sl - 1;
-	const int self = (layer + 1) * axis_stride(A) + (u + 1) * axis_stride(u_axis(A)) + (v + 1) * axis_stride(v_axis(A));
-	const uint2 l = light4(s.vox, face, self);
-	const uint32_t tc = row_code(s, s.vox[self] >> kVoxPidShift, face);
-	s.fs.key[i] = make_uint2(l.x, l.y | (tc << 24));
-	if ((s.codes[tc] >> 16) & kCodeTransPass)
-		atomicOr(reinterpret_cast<uint32_t *>(&s.tp[t][u & ~1]), 1u << (v + 16 * (u & 1)));
-}
-
-// Equal-to-right and equal-to-below bits of visible face i (its neighbours' dense indices follow from
-// the visible masks, since faces are numbered row by row).
-template <int NT> __device__ __forceinline__ void face_links(Smem<NT> &s, int i) {
-	const uint32_t d = s.fs.desc[i];
-	const int t = d & 127, u = (d >> 7) & 15, v = (d >> 11) & 15;
-	const uint32_t ne = s.ne[t][u];
-	const uint2 k = s.fs.key[i];
-	if (v < 15 && ((ne >> (v + 1)) & 1u)) {
-		const uint2 kr = s.fs.key[i + 1];
-		if (kr.x == k.x && kr.y == k.y)
-			atomicOr(reinterpret_cast<uint32_t *>(&s.eh[t][u & ~1]), 1u << (v + 16 * (u & 1)));
+	// class bit 0 = opaque or liquid, bit 1 = transparent or liquid
+	const uint32_t q0 = m0 >> 1, q1 = m1 >> 1; // the - face of the voxel at a + 1, at bit a
+	const uint32_t Ep = ~p0 & ~p1, Op = p0 & ~p1, Tp = ~p0 & p1, Lp = p0 & p1;
+	const uint32_t Eq = ~q0 & ~q1, Oq = q0 & ~q1, Tq = ~q0 & q1, Lq = q0 & q1;
+	uint32_t vp = ((Op & ~Oq) | (Tp & (Eq | Lq)) | (Lp & Eq)) & 0x1fffeu; // + face of a, owner a in [1, 16]
+	uint32_t vm = ((Oq & ~Op) | (Tq & (Ep | Lp)) | (Lq & Ep)) & 0xffffu;  // - face of a + 1, owner in [1, 16]
+	// transparent against transparent: visible iff the two textures differ (the same pair of codes for
+	// both directions)
+	uint32_t tt = (Tp & Tq) & 0x1ffffu;
+	while (tt) {
+		const int a = __ffs(tt) - 1;
+		tt &= tt - 1;
+		const uint32_t wa = s.vox[base + a * sa], wb = s.vox[base + (a + 1) * sa];
+		if (row_code(s, wa >> kVoxPidShift, 2 * A) != row_code(s, wb >> kVoxPidShift, 2 * A + 1))
+			vp |= (1u << a) & 0x1fffeu, vm |= (1u << a) & 0xffffu;
 	}
-	if (u < 15) {
-		const uint32_t nd = s.ne[t][u + 1];
-		if ((nd >> v) & 1u) {
-			const int j = i + 1 + __popc(ne >> (v + 1)) + __popc(nd & ((1u << v) - 1u));
-			const uint2 kd = s.fs.key[j];
-			if (kd.x == k.x && kd.y == k.y)
-				atomicOr(reinterpret_cast<uint32_t *>(&s.ev[t][u & ~1]), 1u << (v + 16 * (u & 1)));
+	const int q = (col >> 6) & 3; // this wave's group of four mask rows
+#pragma unroll
+	for (int a = 0; a <= 16; ++a) {
+		const uint64_t bp = ballot64((vp >> a) & 1u), bm = ballot64((vm >> a) & 1u);
+		if (lane == 0) {
+			reinterpret_cast<uint64_t *>(s.ne[34 * A + 2 * a])[q] = bp;
+			reinterpret_cast<uint64_t *>(s.ne[34 * A + 2 * a + 1])[q] = bm;
 		}
 	}
+}
+
+// Halo index of the block owning face (task t, row u, column v), and its face.
+__device__ __forceinline__ int face_owner(int t, int u, int v, int &face) {
+	const int A = t / 34, sl = (t % 34) >> 1, inv = t & 1;
+	face = 2 * A + inv;
+	const int layer = inv ? sl : sl - 1;
+	return (layer + 1) * axis_stride(A) + (u + 1) * axis_stride(u_axis(A)) + (v + 1) * axis_stride(v_axis(A));
+}
+
+// Word of the visible face with dense index gi (task-major, row-major, as the reference scans them):
+// Light4 (light4_init, :48-163) in x and the low 24 bits of y, the texture code in y bits 24-30 and the
+// transparent-pass flag in bit 31. Faces merge only if their words are equal (Light4::operator==, :44,
+// and the texture compare, :367-368).
+template <int NT> __device__ __forceinline__ uint2 face_word(const Smem<NT> &s, int t0, int t1, uint32_t gi) {
+	int lo = t0, hi = t1 - 1; // the task holding gi: tpre[t] <= gi < tpre[t + 1]
+	while (lo < hi) {
+		const int mid = (lo + hi + 1) >> 1;
+		if (s.tpre[mid] <= gi)
+			lo = mid;
+		else
+			hi = mid - 1;
+	}
+	const int t = lo;
+	uint32_t rem = gi - s.tpre[t];
+	const uint64_t *rows = reinterpret_cast<const uint64_t *>(s.ne[t]);
+	int u = 0;
+	uint32_t bits = 0;
+#pragma unroll 1
+	for (int q = 0; q < 4; ++q) { // the row holding it
+		const uint64_t w = rows[q];
+		const uint32_t c = __popcll(w);
+		if (rem >= c) {
+			rem -= c;
+			continue;
+		}
+#pragma unroll
+		for (int r = 0; r < 4; ++r) {
+			const uint32_t row = uint32_t(w >> (16 * r)) & 0xffffu, cr = __popc(row);
+			if (bits == 0 && rem < cr)
+				u = 4 * q + r, bits = row;
+			else if (bits == 0)
+				rem -= cr;
+		}
+		break;
+	}
+	// the column: the rem-th set bit of the row
+	int v = 0;
+	const uint32_t c8 = __popc(bits & 0xffu);
+	if (rem >= c8)
+		rem -= c8, bits >>= 8, v += 8;
+	const uint32_t c4 = __popc(bits & 0xfu);
+	if (rem >= c4)
+		rem -= c4, bits >>= 4, v += 4;
+	const uint32_t c2 = __popc(bits & 0x3u);
+	if (rem >= c2)
+		rem -= c2, bits >>= 2, v += 2;
+	if (rem >= (bits & 1u))
+		v += 1;
+	int face;
+	const int self = face_owner(t, u, v, face);
+	const uint2 l = light4(s.vox, face, self);
+	const uint32_t tc = row_code(s, s.vox[self] >> kVoxPidShift, face);
+	const uint32_t tpass = (s.codes[tc] >> 16) & kCodeTransPass ? 1u : 0u;
+	return make_uint2(l.x, l.y | (tc << 24) | (tpass << 31));
+}
+
+// Equal-to-right, equal-to-below and transparent-pass masks of mask row (t, u) from the face words of
+// rows u and u + 1 (consecutive in the dense order), without atomics.
+template <int NT> __device__ __forceinline__ void row_links(Smem<NT> &s, int t, int u, uint32_t g0) {
+	const uint32_t ne_u = s.ne[t][u];
+	uint32_t eh = 0, ev = 0, tp = 0;
+	if (ne_u) {
+		const uint64_t *rows = reinterpret_cast<const uint64_t *>(s.ne[t]);
+		uint32_t above = 0; // faces in rows 0 .. u - 1
+#pragma unroll
+		for (int q = 0; q < 4; ++q) {
+			const uint64_t w = rows[q];
+			const int nb = 16 * (u - 4 * q); // bits of this word in rows above u
+			above += nb >= 64 ? __popcll(w) : (nb > 0 ? __popcll(w & ((1ull << nb) - 1ull)) : 0u);
+		}
+		const uint32_t i0 = s.tpre[t] - g0 + above, i1 = i0 + __popc(ne_u);
+		const uint32_t ne_d = u < 15 ? s.ne[t][u + 1] : 0u;
+		uint32_t bits = ne_u, k = 0;
+		while (bits) {
+			const uint32_t v = __ffs(bits) - 1;
+			bits &= bits - 1;
+			const uint2 w = s.key[i0 + k];
+			tp |= (w.y >> 31) << v;
+			if (v < 15 && ((ne_u >> (v + 1)) & 1u)) {
+				const uint2 r = s.key[i0 + k + 1];
+				eh |= uint32_t(r.x == w.x && r.y == w.y) << v;
+			}
+			if ((ne_d >> v) & 1u) {
+				const uint2 d = s.key[i1 + __popc(ne_d & ((1u << v) - 1u))];
+				ev |= uint32_t(d.x == w.x && d.y == w.y) << v;
+			}
+			++k;
+		}
+	}
+	s.eh[t][u] = uint16_t(eh);
+	s.ev[t][u] = uint16_t(ev);
+	s.tp[t][u] = uint16_t(tp);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -678,9 +765,8 @@ __device__ __forceinline__ uint32_t spread_up(uint32_t f, uint32_t link) {
 	return f;
 }
 
-template <int NT, bool EMIT, typename Fn>
-__device__ __forceinline__ void greedy_task(const Smem<NT> &s, int t, bool active, uint32_t &n_o, uint32_t &n_t,
-                                            Fn &&emit) {
+template <int NT>
+__device__ __forceinline__ void greedy_count(Smem<NT> &s, int t, bool active, uint32_t &n_o, uint32_t &n_t) {
 	// coverage of the rows below the current one by quads of earlier rows, and the equal-to-below masks,
 	// held in registers as windows that shift up by one row per step (window row k = row u + k)
 	uint32_t cov[8], ev[8];
@@ -691,9 +777,12 @@ __device__ __forceinline__ void greedy_task(const Smem<NT> &s, int t, bool activ
 		ev[k] = active ? evs[k] : 0u;
 	}
 	n_o = 0, n_t = 0;
+#pragma unroll 1
 	for (int u = 0; u < 16; ++u) {
 		const uint32_t ne = active ? s.ne[t][u] : 0u, eh = active ? s.eh[t][u] : 0u, tp = active ? s.tp[t][u] : 0u;
 		const uint32_t S = ne & ~(cov[0] & 0xffffu); // uncovered visible cells of row u
+		if (active)
+			s.ev[t][u] = uint16_t(S); // kept for the emission pass (this row of ev is in registers)
 		const uint32_t link = S & ((S & eh) << 1);   // v continues the run of v - 1
 		const uint32_t starts = S & ~link;
 		if (__ballot(S != 0)) {
@@ -710,25 +799,10 @@ __device__ __forceinline__ void greedy_task(const Smem<NT> &s, int t, bool activ
 					ok = u + k < 16 ? (evr & ~cvr) : 0u;
 				}
 				const uint32_t bad = cols & ~ok;
-				uint32_t dead = 0;
 				if (bad) {
-					dead = spread_down(bad, link) & alive;
+					const uint32_t dead = spread_down(bad, link) & alive;
 					alive &= ~dead;
 					cols &= ~spread_up(dead, link);
-				}
-				if (EMIT) {
-					while (dead) {
-						const uint32_t v0 = __ffs(dead) - 1;
-						dead &= dead - 1;
-						const uint32_t below = (1u << v0) - 1u;
-						const uint32_t tr = (tp >> v0) & 1u;
-						Quad q;
-						q.u = u, q.v0 = v0, q.h = k, q.trans = tr;
-						q.w = __ffs(~(link >> (v0 + 1)));
-						q.ord = n_o + n_t + __popc(starts & below);
-						q.ord_pass = tr ? n_t + __popc(starts & tp & below) : n_o + __popc(starts & ~tp & below);
-						emit(q);
-					}
 				}
 				if (k < 16)
 					cov[k >> 1] |= cols << (16 * (k & 1));
@@ -744,6 +818,51 @@ __device__ __forceinline__ void greedy_task(const Smem<NT> &s, int t, bool activ
 			cov[k] = (cov[k] >> 16) | (nc << 16);
 			ev[k] = (ev[k] >> 16) | (nv << 16);
 		}
+	}
+}
+
+// Emission pass of task t's merge from the rows' uncovered sets S left by the counting pass. The quads
+// starting in row u are S's runs split where the face word changes. A quad from (u, v0) covers rows
+// u .. u + h - 1 of its columns, so below it column v0 is covered (visible, not in S) exactly for
+// rows u + 1 .. u + h - 1: the cell under its last row is either not covered or starts a quad.
+template <int NT, typename Fn>
+__device__ __forceinline__ void greedy_emit(const Smem<NT> &s, int t, bool active, Fn &&emit) {
+	uint32_t n_o = 0, n_t = 0;
+#pragma unroll 1
+	for (int u = 0; u < 16; ++u) {
+		const uint32_t S = active ? s.ev[t][u] : 0u, eh = active ? s.eh[t][u] : 0u, tp = active ? s.tp[t][u] : 0u;
+		const uint32_t link = S & ((S & eh) << 1);
+		const uint32_t starts = S & ~link;
+		if (!__ballot(starts != 0))
+			continue;
+		uint32_t alive = starts, hb0 = 0, hb1 = 0, hb2 = 0, hb3 = 0; // bit planes of (height - 1)
+#pragma unroll 1
+		for (int k = 1; u + k < 16; ++k) {
+			alive &= active ? (uint32_t(s.ne[t][u + k]) & ~uint32_t(s.ev[t][u + k])) : 0u;
+			if (!__ballot(alive != 0))
+				break;
+			hb0 = (k & 1) ? (hb0 | alive) : (hb0 & ~alive);
+			hb1 = (k & 2) ? (hb1 | alive) : (hb1 & ~alive);
+			hb2 = (k & 4) ? (hb2 | alive) : (hb2 & ~alive);
+			hb3 = (k & 8) ? (hb3 | alive) : (hb3 & ~alive);
+		}
+		uint32_t st = starts;
+		while (st) {
+			const uint32_t v0 = __ffs(st) - 1;
+			st &= st - 1;
+			const uint32_t below = (1u << v0) - 1u;
+			const uint32_t tr = (tp >> v0) & 1u;
+			Quad q;
+			q.u = u, q.v0 = v0, q.trans = tr;
+			q.h = 1u + ((hb0 >> v0) & 1u) + 2u * ((hb1 >> v0) & 1u) + 4u * ((hb2 >> v0) & 1u) + 8u * ((hb3 >> v0) & 1u);
+			q.w = __ffs(~(link >> (v0 + 1)));
+			q.ord = n_o + n_t + __popc(starts & below);
+			q.ord_pass = tr ? n_t + __popc(starts & tp & below) : n_o + __popc(starts & ~tp & below);
+			emit(q);
+		}
+		const uint32_t nt = __popc(starts & tp);
+		n_t += nt;
+		n_o += __popc(starts) - nt;
 	}
 }
 
@@ -845,20 +964,23 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 		__syncthreads();
 	}
 
-	// 3. visibility masks; zero the link masks
-	for (int t = wave; t < kTasks; t += NT / 64) {
-		const uint32_t n = task_visibility(s, t);
-		if (lane == 0)
-			s.tpre[t] = n;
-	}
-	for (int i = tid; i < kTasks * 8; i += NT) {
-		reinterpret_cast<uint32_t *>(s.eh)[i] = 0;
-		reinterpret_cast<uint32_t *>(s.ev)[i] = 0;
-		reinterpret_cast<uint32_t *>(s.tp)[i] = 0;
-	}
+	// 3. faces: visibility masks by bit slices; then, for each group of tasks whose visible faces fit the
+	// stage, the face words (one lane per face, in dense order) and the link masks (one lane per row)
+#pragma unroll 1
+	for (int A = 0; A < 3; ++A)
+		for (int col = tid; col < 256; col += NT)
+			visibility_column(s, A, col);
 	__syncthreads();
 	if (wave == 0) { // exclusive prefix of visible faces over the tasks
-		const uint32_t a = lane < kTasks ? s.tpre[lane] : 0u, b = lane + 64 < kTasks ? s.tpre[lane + 64] : 0u;
+		uint32_t a = 0, b = 0;
+		if (lane < kTasks) {
+			const uint64_t *r = reinterpret_cast<const uint64_t *>(s.ne[lane]);
+			a = __popcll(r[0]) + __popcll(r[1]) + __popcll(r[2]) + __popcll(r[3]);
+		}
+		if (lane + 64 < kTasks) {
+			const uint64_t *r = reinterpret_cast<const uint64_t *>(s.ne[lane + 64]);
+			b = __popcll(r[0]) + __popcll(r[1]) + __popcll(r[2]) + __popcll(r[3]);
+		}
 		uint32_t ia = a, ib = b;
 #pragma unroll
 		for (int d = 1; d < 64; d <<= 1) {
@@ -875,106 +997,57 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 			s.tpre[kTasks] = tot_a + ib;
 	}
 	__syncthreads();
-
-	// 4. light words for the visible faces, a group of whole tasks at a time
+#pragma unroll 1
 	for (int t0 = 0; t0 < kTasks;) {
-		if (tid == 0) { // the longest run of tasks whose faces fit the stage (one task always fits)
+		if (tid == 0) { // the longest run of tasks whose faces fit (one task, at most 256 faces, always fits)
 			int t1 = t0 + 1;
-			while (t1 < kTasks && s.tpre[t1 + 1] - s.tpre[t0] <= (uint32_t)kFaceCap)
+			while (t1 < kTasks && s.tpre[t1 + 1] - s.tpre[t0] <= (uint32_t)kListCap)
 				++t1;
 			s.group_end = t1;
 		}
 		__syncthreads();
 		const int t1 = s.group_end;
-		const uint32_t base = s.tpre[t0], n = s.tpre[t1] - base;
-		for (int pr = tid; pr < (t1 - t0) * 16; pr += NT) { // number the faces row by row
-			const int t = t0 + pr / 16, u = pr % 16;
-			uint32_t bits = s.ne[t][u];
-			if (!bits)
-				continue;
-			uint32_t at = s.tpre[t] - base;
-			for (int k = 0; k < u; ++k)
-				at += __popc(s.ne[t][k]);
-			while (bits) {
-				const uint32_t v = __ffs(bits) - 1;
-				bits &= bits - 1;
-				s.fs.desc[at++] = uint32_t(t) | uint32_t(u) << 7 | v << 11;
-			}
-		}
-		__syncthreads();
+		const uint32_t g0 = s.tpre[t0], n = s.tpre[t1] - g0;
 		for (uint32_t i = tid; i < n; i += NT)
-			face_key(s, i);
+			s.key[i] = face_word(s, t0, t1, g0 + i);
 		__syncthreads();
-		for (uint32_t i = tid; i < n; i += NT)
-			face_links(s, i);
+		for (int item = tid; item < (t1 - t0) * 16; item += NT)
+			row_links(s, t0 + (item >> 4), item & 15, g0);
 		__syncthreads();
 		t0 = t1;
 	}
 
-	// 5. counts: custom faces per thread range of cells, greedy quads per task
-	uint32_t c_o = 0, c_t = 0;
-	if (flags & kFlagCustom) {
-		for (int k = 0; k < CPT; ++k) {
-			const int cell = tid * CPT + k;
-			if (!((s.custom_bits[cell >> 5] >> (cell & 31)) & 1u))
-				continue;
-			const CustomCell c = custom_cell(p, s, cell);
-			const uint32_t nt = __popc(c.trans);
-			c_t += nt;
-			c_o += c.count - nt;
-		}
-	}
-	uint32_t q_o = 0, q_t = 0;
-	if (wave * 64 < kTasks)
-		greedy_task<NT, false>(s, tid, tid < kTasks, q_o, q_t, [](const Quad &) {});
-	uint64_t cust_total, task_total;
-	const uint64_t cust_pre = block_excl_scan(s, (uint64_t)c_o | ((uint64_t)c_t << 32), &cust_total);
-	const uint64_t task_pre = block_excl_scan(s, (uint64_t)q_o | ((uint64_t)q_t << 32), &task_total);
-	const uint32_t cust0 = (uint32_t)cust_total, cust1 = (uint32_t)(cust_total >> 32);
-	const uint32_t n0 = cust0 + (uint32_t)task_total, n1 = cust1 + (uint32_t)(task_total >> 32);
-	const uint32_t total = n0 + n1;
-	if (total == 0) {
-		if (tid == 0)
-			p.results[job] = hcm_job_result{HCM_OK, 0u, 0u, 0u};
-		return;
-	}
-
-	// 6. allocation: the chunk's quads [pool_base, +n0) opaque, [+n0, +n0+n1) transparent
-	if (tid == 0) {
-		const uint32_t nm0 = (n0 + kSplitQuads - 1) / kSplitQuads, nm1 = (n1 + kSplitQuads - 1) / kSplitQuads;
-		uint32_t status = HCM_OK;
-		const uint32_t pbase = atomicAdd(&p.ctr->pool_cursor, total);
-		if ((uint64_t)pbase + total > p.pool_cap)
-			status = (uint32_t)HCM_ERR_POOL_FULL;
-		const uint32_t mbase = atomicAdd(&p.ctr->mesh_cursor, nm0 + nm1);
-		if ((uint64_t)mbase + nm0 + nm1 > p.rec_cap)
-			status = (uint32_t)HCM_ERR_POOL_FULL;
-		s.status = status;
-		s.pool_base = pbase;
-		s.mesh_base = mbase;
-		s.n_mesh0 = nm0;
-		s.n_mesh1 = nm1;
-		if (status != HCM_OK)
-			p.results[job] = hcm_job_result{(int32_t)status, 0u, 0u, total};
-	}
-	__syncthreads();
-	if (s.status != HCM_OK)
-		return;
-	const uint32_t pool_base = s.pool_base;
-
-	// 7. describe quads into the LDS list window by window, then write them one lane per quad
-	const uint32_t task_o = cust0 + (uint32_t)task_pre, task_t = cust1 + (uint32_t)(task_pre >> 32);
-	const uint32_t task_g = cust0 + cust1 + (uint32_t)task_pre + (uint32_t)(task_pre >> 32);
-	for (uint32_t w0 = 0; w0 < total; w0 += kListCap) {
+	// 4-6. one rolled loop (one copy of the greedy merge and of the custom-cell walk in the code):
+	// pass 0 counts the quads -- custom faces per thread range of cells, greedy quads per task -- and
+	// allocates the chunk's output: [pool_base, +n0) opaque, [+n0, +n0+n1) transparent. Each later
+	// pass describes the quads of one window of the chunk's emission order into the LDS list and
+	// writes them, one lane per quad.
+	uint32_t c_o = 0, c_t = 0, q_o = 0, q_t = 0;
+	uint64_t cust_pre = 0, task_pre = 0;
+	uint32_t n0 = 0, n1 = 0, total = 0, pool_base = 0, task_o = 0, task_t = 0, task_g = 0;
+	// AABB<uint32_t>::Merge (dep/AABB.hpp:52-59) of the first mesh of each pass, per lane, reduced at the end
+	uint32_t box[2][6];
+#pragma unroll
+	for (int a = 0; a < 6; ++a)
+		box[0][a] = box[1][a] = a < 3 ? ~0u : 0u;
+#pragma unroll 1
+	for (uint32_t gp = 0, w0 = 0;; ++gp) {
+		const bool counting = gp == 0;
 		const uint32_t w1 = w0 + kListCap;
 		if (flags & kFlagCustom) {
 			uint32_t io = (uint32_t)cust_pre, it = (uint32_t)(cust_pre >> 32), g = io + it;
-			if (g < w1) {
+			if (counting || g < w1) {
 				for (int k = 0; k < CPT; ++k) {
 					const int cell = tid * CPT + k;
 					if (!((s.custom_bits[cell >> 5] >> (cell & 31)) & 1u))
 						continue;
 					const CustomCell c = custom_cell(p, s, cell);
+					if (counting) {
+						const uint32_t nt = __popc(c.trans);
+						c_t += nt;
+						c_o += c.count - nt;
+						continue;
+					}
 					for (uint32_t j = 0; j < c.count; ++j, ++g) {
 						const uint32_t tr = (c.trans >> j) & 1u, i = tr ? it++ : io++;
 						if (g >= w0 && g < w1)
@@ -984,40 +1057,119 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 			}
 		}
 		if (wave * 64 < kTasks) {
-			const bool mine = tid < kTasks && task_g < w1 && task_g + q_o + q_t > w0;
-			uint32_t eo, et;
-			greedy_task<NT, true>(s, tid, mine, eo, et, [&](const Quad &q) {
-				const uint32_t g = task_g + q.ord;
-				if (g >= w0 && g < w1) {
-					const uint32_t i = (q.trans ? task_t : task_o) + q.ord_pass;
-					s.list[g - w0] = make_uint2(uint32_t(tid) | (q.u << 7) | (q.v0 << 11) | ((q.w - 1) << 15) |
-					                                ((q.h - 1) << 19),
-					                            i | (q.trans << 31));
-				}
-			});
+			const bool mine = tid < kTasks && (counting || (task_g < w1 && task_g + q_o + q_t > w0));
+			if (counting)
+				greedy_count(s, tid, mine, q_o, q_t);
+			else
+				greedy_emit(s, tid, mine, [&](const Quad &q) {
+					const uint32_t g = task_g + q.ord;
+					if (g >= w0 && g < w1) {
+						const uint32_t i = (q.trans ? task_t : task_o) + q.ord_pass;
+						s.list[g - w0] = make_uint2(uint32_t(tid) | (q.u << 7) | (q.v0 << 11) | ((q.w - 1) << 15) |
+						                                ((q.h - 1) << 19),
+						                            i | (q.trans << 31));
+					}
+				});
 		}
+		if (counting) {
+			uint64_t cust_total, task_total;
+			cust_pre = block_excl_scan(s, (uint64_t)c_o | ((uint64_t)c_t << 32), &cust_total);
+			task_pre = block_excl_scan(s, (uint64_t)q_o | ((uint64_t)q_t << 32), &task_total);
+			const uint32_t cust0 = (uint32_t)cust_total, cust1 = (uint32_t)(cust_total >> 32);
+			n0 = cust0 + (uint32_t)task_total, n1 = cust1 + (uint32_t)(task_total >> 32);
+			total = n0 + n1;
+			if (total == 0) {
+				if (tid == 0)
+					p.results[job] = hcm_job_result{HCM_OK, 0u, 0u, 0u};
+				return;
+			}
+			task_o = cust0 + (uint32_t)task_pre, task_t = cust1 + (uint32_t)(task_pre >> 32);
+			task_g = cust0 + cust1 + (uint32_t)task_pre + (uint32_t)(task_pre >> 32);
+			// 5. allocation
+			if (tid == 0) {
+				const uint32_t nm0 = (n0 + kSplitQuads - 1) / kSplitQuads, nm1 = (n1 + kSplitQuads - 1) / kSplitQuads;
+				uint32_t status = HCM_OK;
+				const uint32_t pbase = atomicAdd(&p.ctr->pool_cursor, total);
+				if ((uint64_t)pbase + total > p.pool_cap)
+					status = (uint32_t)HCM_ERR_POOL_FULL;
+				const uint32_t mbase = atomicAdd(&p.ctr->mesh_cursor, nm0 + nm1);
+				if ((uint64_t)mbase + nm0 + nm1 > p.rec_cap)
+					status = (uint32_t)HCM_ERR_POOL_FULL;
+				s.status = status;
+				s.pool_base = pbase;
+				s.mesh_base = mbase;
+				s.n_mesh0 = nm0;
+				s.n_mesh1 = nm1;
+				if (status != HCM_OK)
+					p.results[job] = hcm_job_result{(int32_t)status, 0u, 0u, total};
+			}
+			__syncthreads();
+			if (s.status != HCM_OK)
+				return;
+			pool_base = s.pool_base;
+			continue;
+		}
+		// 6. write the window's quads
 		__syncthreads();
 		const uint32_t n = min(total - w0, (uint32_t)kListCap);
 		for (uint32_t e = tid; e < n; e += NT) {
 			const uint2 ent = s.list[e];
 			const uint32_t pass = ent.y >> 31, i = ent.y & 0x7fffffffu;
 			const uint32_t slot = pool_base + (pass ? n0 : 0u) + i;
+			const uint32_t mk = i / kSplitQuads, mj = i - mk * kSplitQuads; // mesh of the pass, index in it
+			// the reference reads the 16-bit vertex count before splitting, so the first quad of a
+			// continuation mesh keeps base 65,532 (:273-279, :397-403)
+			const uint32_t vbase = (mj == 0 && mk > 0) ? 65532u : 4u * mj;
+			uint32_t lo[3], hi[3];
 			if (ent.x & 0x80000000u)
-				emit_custom(p, s, ent.x, slot, pass, i);
+				emit_custom(p, s, ent.x, slot, vbase, lo, hi);
 			else
-				emit_regular(p, s, ent.x, slot, pass, i);
-			if (i > 0 && i % kSplitQuads == 0) { // a split happened right before this quad (:397-403)
+				emit_regular(p, s, ent.x, slot, vbase, lo, hi);
+			if (mk == 0) { // the pass's first mesh: this lane's running box (static indices only)
+#pragma unroll
+				for (int a = 0; a < 3; ++a) {
+					box[0][a] = pass ? box[0][a] : min(box[0][a], lo[a]);
+					box[0][3 + a] = pass ? box[0][3 + a] : max(box[0][3 + a], hi[a]);
+					box[1][a] = pass ? min(box[1][a], lo[a]) : box[1][a];
+					box[1][3 + a] = pass ? max(box[1][3 + a], hi[a]) : box[1][3 + a];
+				}
+			} else
+				aabb_merge(s, pass, mk, lo, hi);
+			if (mj == 0 && mk > 0) { // a split happened right before this quad (:397-403)
 				const uint32_t ev = atomicAdd(&s.n_events, 1u);
 				if (ev < 16) {
 					s.events[ev][0] = w0 + e;
-					s.events[ev][1] = (pass << 16) | (i / kSplitQuads - 1u);
+					s.events[ev][1] = (pass << 16) | (mk - 1u);
 				}
 			}
 		}
 		__syncthreads();
+		w0 += kListCap;
+		if (w0 >= total)
+			break;
 	}
 
-	// 8. mesh records in the reference's order: split meshes by time, then opaque, then transparent
+#pragma unroll
+	for (int pass = 0; pass < 2; ++pass)
+#pragma unroll
+		for (int a = 0; a < 6; ++a) {
+			uint32_t b = box[pass][a];
+#pragma unroll
+			for (int d = 1; d < 64; d <<= 1) {
+				const uint32_t o = __shfl_xor(b, d);
+				b = a < 3 ? min(b, o) : max(b, o);
+			}
+			box[pass][a] = b;
+		}
+	if (lane == 0 && (n0 | n1)) {
+		if (n0)
+			aabb_merge(s, 0, 0, box[0], box[0] + 3);
+		if (n1)
+			aabb_merge(s, 1, 0, box[1], box[1] + 3);
+	}
+	__syncthreads();
+
+	// 7. mesh records in the reference's order: split meshes by time, then opaque, then transparent
 	if (tid == 0) {
 		const uint32_t ne = min(s.n_events, 16u);
 		for (uint32_t a = 1; a < ne; ++a) // insertion sort by emission position
@@ -1049,7 +1201,7 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 	}
 }
 
-template <int NT> __global__ __launch_bounds__(NT) void mesh_kernel(MeshParams p) {
+template <int NT> __global__ __launch_bounds__(NT, NT >= 512 ? 2 : 3) void mesh_kernel(MeshParams p) {
 	__shared__ Smem<NT> s;
 	for (int i = threadIdx.x; i < kMaxPid; i += NT) {
 		const bool ok = i < (int)p.reg.n_pid;

@@ -4,8 +4,10 @@
 // hc::client::BlockMeshAlgo<Config>::Generate (client/include/client/BlockMeshAlgo.hpp:494-521) for a
 // 16^3 chunk and returns the same std::vector<BlockMesh> (client/include/client/BlockVertex.hpp:8-25):
 // packed vertices, u16 indices, AABB<uint32_t> in 1/16 block units, and the transparent flag, in the
-// reference's mesh order. The callables are evaluated over the 48^3 voxels of the chunk and its 26
-// neighbours (x, y, z in [-16, 32)); the meshing itself runs on the GPU through include/hcm.h.
+// reference's mesh order. The callables are evaluated only where the mesher reads: blocks over
+// x, z in [-2, 18), y in [-1, 17) (the 18^3 halo plus the x/z cells flowing water reads around halo
+// cells, BlockMeshAlgo.hpp:202-236), light over the 18^3 halo [-1, 17)^3; the meshing itself runs on
+// the GPU through include/hcm.h.
 #pragma once
 
 #include <array>
@@ -59,20 +61,22 @@ public:
 	// 16-bit Block::m_data (e.g. via .GetData()) and get_light(x, y, z) the 8-bit Light::m_data.
 	template <typename GetBlockFunc, typename GetLightFunc>
 	std::vector<BlockMesh> Generate(GetBlockFunc &&get_block_func, GetLightFunc &&get_light_func) {
-		for (int y = -16; y < 32; ++y)
-			for (int z = -16; z < 32; ++z)
-				for (int x = -16; x < 32; ++x) {
+		for (int y = -1; y < 17; ++y)
+			for (int z = -2; z < 18; ++z)
+				for (int x = -2; x < 18; ++x) {
 					const int slot = slot_of(x, y, z);
 					const int local = (x & 15) + 16 * ((z & 15) + 16 * (y & 15));
-					m_blocks[slot][local] = data_of(get_block_func(x, y, z));
-					m_light[slot][local] = (uint8_t)data_of(get_light_func(x, y, z));
+					m_blocks[slot][local] = (uint16_t)data_of(get_block_func(x, y, z));
+					const bool halo = x >= -1 && x < 17 && z >= -1 && z < 17;
+					m_light[slot][local] = halo ? (uint8_t)data_of(get_light_func(x, y, z)) : 0;
 				}
-		return Generate(m_block_ptrs, m_light_ptrs);
+		return GenerateChunks(m_block_ptrs, m_light_ptrs);
 	}
 
 	// The 27 chunks directly, in neighbour-slot order (common/include/common/Position.hpp:138-145);
-	// e.g. neighbour_chunks[i]->GetBlockData() at client/src/ChunkMeshTask.cpp:117-119.
-	std::vector<BlockMesh> Generate(const uint16_t *const blocks[27], const uint8_t *const light[27]) {
+	// e.g. neighbour_chunks[i]->GetBlockData() at client/src/ChunkMeshTask.cpp:117-119. (A distinct name:
+	// arrays of pointers would otherwise bind to the callable overload above.)
+	std::vector<BlockMesh> GenerateChunks(const uint16_t *const blocks[27], const uint8_t *const light[27]) {
 		hcm_host_result out{};
 		for (;;) {
 			out.vertices = reinterpret_cast<uint32_t(*)[2]>(m_verts.data());

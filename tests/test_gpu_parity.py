@@ -181,3 +181,19 @@ def test_pool_full_is_reported():
         assert set(st.tolist()) <= {0, hcm.HCM_ERR_POOL_FULL}
     finally:
         small.close()
+
+
+def test_cpp_front_end_generate(golden, tmp_path):
+    """hcm::BlockMeshGPU::Generate(get_block, get_light) (include/hcm.hpp), called like the reference's
+    BlockMeshAlgo::Generate with callables returning GetData() types, gives the reference's meshes."""
+    import subprocess
+
+    import test_library
+    from hypercraft_amd import build as hb
+
+    say("C++ front end: terrain chunk through BlockMeshGPU::Generate")
+    out = tmp_path / "meshes.bin"
+    r = subprocess.run([hb.WRAPPER_CHECK, str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = test_library.read_wrapper_output(out)
+    assert meshdigest.digest(got) == golden["terrain_chunk_digest"][0].tobytes()

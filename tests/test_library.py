@@ -52,3 +52,37 @@ def test_random_dense_mix():
     op = np.isin(v, list(opaque)).mean()
     water = (v == 14).mean()
     assert abs(air - 0.50) < 0.01 and abs(op - 0.35) < 0.01 and abs(water - 0.05) < 0.005
+
+
+def read_wrapper_output(path):
+    """Mesh list written by tests/cpp/wrapper_check.cpp."""
+    from hypercraft_amd.hcm import Mesh
+
+    data = open(path, "rb").read()
+    n = int(np.frombuffer(data[:4], np.uint32)[0])
+    at, out = 4, []
+    for _ in range(n):
+        hdr = np.frombuffer(data[at:at + 36], np.uint32)
+        at += 36
+        nv, ni = int(hdr[0]), int(hdr[1])
+        verts = np.frombuffer(data[at:at + 8 * nv], np.uint32).reshape(nv, 2)
+        at += 8 * nv
+        idx = np.frombuffer(data[at:at + 2 * ni], np.uint16)
+        at += 2 * ni
+        out.append(Mesh(verts, idx, tuple(int(a) for a in hdr[2:8]), int(hdr[8])))
+    assert at == len(data)
+    return out
+
+
+def test_cpp_front_end_builds_and_fails_cleanly_without_a_device(tmp_path):
+    """include/hcm.hpp compiles into a program (built with the library) that meshes through the
+    reference's call shape; with no gfx950 device it reports HCM_ERR_NO_DEVICE instead of meshing."""
+    import subprocess
+
+    from hypercraft_amd import build as hb
+
+    assert os.path.exists(hb.WRAPPER_CHECK), "built by hypercraft_amd/build.py"
+    r = subprocess.run([hb.WRAPPER_CHECK, str(tmp_path / "m.bin")], capture_output=True, text=True, timeout=120)
+    assert r.returncode in (0, 3), r.stderr
+    if r.returncode == 3:
+        assert "-5" in r.stderr
