@@ -111,7 +111,8 @@ template <int NT> struct Smem {
 		uint2 key[kListCap];  // face words of the current group of tasks, in dense order
 		uint2 list[kListCap]; // quad descriptors of the current emission window
 	};
-	uint32_t tpre[kTasks + 2]; // prefix of visible faces per task
+	uint32_t tpre[kTasks + 2];  // prefix of visible faces per task; later of quads in emission order
+	uint32_t qpre_o[kTasks];     // per task: index of its first opaque quad in the opaque pass
 	uint2 rows[kMaxPid];
 	uint32_t pid_vox[kMaxPid];
 	uint32_t codes[kMaxCodes];
@@ -781,10 +782,14 @@ __device__ __forceinline__ void greedy_count(Smem<NT> &s, int t, bool active, ui
 	for (int u = 0; u < 16; ++u) {
 		const uint32_t ne = active ? s.ne[t][u] : 0u, eh = active ? s.eh[t][u] : 0u, tp = active ? s.tp[t][u] : 0u;
 		const uint32_t S = ne & ~(cov[0] & 0xffffu); // uncovered visible cells of row u
-		if (active)
-			s.ev[t][u] = uint16_t(S); // kept for the emission pass (this row of ev is in registers)
 		const uint32_t link = S & ((S & eh) << 1);   // v continues the run of v - 1
 		const uint32_t starts = S & ~link;
+		if (active) { // row u's results replace its (already read) inputs; ev's rows are in registers
+			s.ne[t][u] = uint16_t(ne & ~S);     // cells covered by quads of earlier rows
+			s.ev[t][u] = uint16_t(S);
+			s.eh[t][u] = uint16_t(starts & ~tp); // opaque quads start here
+			s.tp[t][u] = uint16_t(starts & tp);  // transparent quads start here
+		}
 		if (__ballot(S != 0)) {
 			uint32_t alive = starts, cols = S;
 #pragma unroll
@@ -821,48 +826,48 @@ __device__ __forceinline__ void greedy_count(Smem<NT> &s, int t, bool active, ui
 	}
 }
 
-// Emission pass of task t's merge from the rows' uncovered sets S left by the counting pass. The quads
-// starting in row u are S's runs split where the face word changes. A quad from (u, v0) covers rows
-// u .. u + h - 1 of its columns, so below it column v0 is covered (visible, not in S) exactly for
-// rows u + 1 .. u + h - 1: the cell under its last row is either not covered or starts a quad.
-template <int NT, typename Fn>
-__device__ __forceinline__ void greedy_emit(const Smem<NT> &s, int t, bool active, Fn &&emit) {
-	uint32_t n_o = 0, n_t = 0;
-#pragma unroll 1
-	for (int u = 0; u < 16; ++u) {
-		const uint32_t S = active ? s.ev[t][u] : 0u, eh = active ? s.eh[t][u] : 0u, tp = active ? s.tp[t][u] : 0u;
-		const uint32_t link = S & ((S & eh) << 1);
-		const uint32_t starts = S & ~link;
-		if (!__ballot(starts != 0))
-			continue;
-		uint32_t alive = starts, hb0 = 0, hb1 = 0, hb2 = 0, hb3 = 0; // bit planes of (height - 1)
-#pragma unroll 1
-		for (int k = 1; u + k < 16; ++k) {
-			alive &= active ? (uint32_t(s.ne[t][u + k]) & ~uint32_t(s.ev[t][u + k])) : 0u;
-			if (!__ballot(alive != 0))
+// The quads of mask row (t, u), from what the counting pass left: C (cells covered by quads of earlier
+// rows) in ne, the uncovered set S in ev, and the opaque and transparent quad starts in eh and tp. A run
+// of S continues through every cell that is not a start. A quad from (u, v0) covers rows u .. u + h - 1
+// of its columns, so below it column v0 is in C exactly for rows u + 1 .. u + h - 1: the cell under its
+// last row is either not covered or starts a quad. Quads are numbered within the task in the
+// reference's order (rows, then columns) and within the task's quads of the same pass.
+template <int NT, typename Fn> __device__ __forceinline__ void describe_row(const Smem<NT> &s, int t, int u, Fn &&emit) {
+	const uint32_t so = s.eh[t][u], st_t = s.tp[t][u], starts = so | st_t;
+	if (!starts)
+		return;
+	const uint64_t *o64 = reinterpret_cast<const uint64_t *>(s.eh[t]);
+	const uint64_t *t64 = reinterpret_cast<const uint64_t *>(s.tp[t]);
+	const uint64_t *c64 = reinterpret_cast<const uint64_t *>(s.ne[t]);
+	uint32_t n_o = 0, n_t = 0; // quads of each pass in rows 0 .. u - 1
+	uint64_t c[4];
+#pragma unroll
+	for (int q = 0; q < 4; ++q) {
+		const int nb = 16 * (u - 4 * q);
+		const uint64_t m = nb >= 64 ? ~0ull : (nb > 0 ? (1ull << nb) - 1ull : 0ull);
+		n_o += __popcll(o64[q] & m);
+		n_t += __popcll(t64[q] & m);
+		c[q] = c64[q];
+	}
+	const uint32_t link = uint32_t(s.ev[t][u]) & ~starts;
+	uint32_t rest = starts;
+	while (rest) {
+		const uint32_t v0 = __ffs(rest) - 1;
+		rest &= rest - 1;
+		const uint32_t below = (1u << v0) - 1u;
+		const uint32_t tr = (st_t >> v0) & 1u;
+		uint32_t h = 1;
+		for (int r = u + 1; r < 16; ++r, ++h) {
+			const uint64_t w = r < 4 ? c[0] : (r < 8 ? c[1] : (r < 12 ? c[2] : c[3]));
+			if (!((w >> (16 * (r & 3) + v0)) & 1ull))
 				break;
-			hb0 = (k & 1) ? (hb0 | alive) : (hb0 & ~alive);
-			hb1 = (k & 2) ? (hb1 | alive) : (hb1 & ~alive);
-			hb2 = (k & 4) ? (hb2 | alive) : (hb2 & ~alive);
-			hb3 = (k & 8) ? (hb3 | alive) : (hb3 & ~alive);
 		}
-		uint32_t st = starts;
-		while (st) {
-			const uint32_t v0 = __ffs(st) - 1;
-			st &= st - 1;
-			const uint32_t below = (1u << v0) - 1u;
-			const uint32_t tr = (tp >> v0) & 1u;
-			Quad q;
-			q.u = u, q.v0 = v0, q.trans = tr;
-			q.h = 1u + ((hb0 >> v0) & 1u) + 2u * ((hb1 >> v0) & 1u) + 4u * ((hb2 >> v0) & 1u) + 8u * ((hb3 >> v0) & 1u);
-			q.w = __ffs(~(link >> (v0 + 1)));
-			q.ord = n_o + n_t + __popc(starts & below);
-			q.ord_pass = tr ? n_t + __popc(starts & tp & below) : n_o + __popc(starts & ~tp & below);
-			emit(q);
-		}
-		const uint32_t nt = __popc(starts & tp);
-		n_t += nt;
-		n_o += __popc(starts) - nt;
+		Quad q;
+		q.u = u, q.v0 = v0, q.trans = tr, q.h = h;
+		q.w = __ffs(~(link >> (v0 + 1)));
+		q.ord = n_o + n_t + __popc(starts & below);
+		q.ord_pass = tr ? n_t + __popc(st_t & below) : n_o + __popc(so & below);
+		emit(q);
 	}
 }
 
@@ -1000,10 +1005,15 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 #pragma unroll 1
 	for (int t0 = 0; t0 < kTasks;) {
 		if (tid == 0) { // the longest run of tasks whose faces fit (one task, at most 256 faces, always fits)
-			int t1 = t0 + 1;
-			while (t1 < kTasks && s.tpre[t1 + 1] - s.tpre[t0] <= (uint32_t)kListCap)
-				++t1;
-			s.group_end = t1;
+			int lo = t0 + 1, hi = kTasks; // largest t1 with tpre[t1] - tpre[t0] <= kListCap
+			while (lo < hi) {
+				const int mid = (lo + hi + 1) >> 1;
+				if (s.tpre[mid] - s.tpre[t0] <= (uint32_t)kListCap)
+					lo = mid;
+				else
+					hi = mid - 1;
+			}
+			s.group_end = lo;
 		}
 		__syncthreads();
 		const int t1 = s.group_end;
@@ -1024,7 +1034,7 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 	// writes them, one lane per quad.
 	uint32_t c_o = 0, c_t = 0, q_o = 0, q_t = 0;
 	uint64_t cust_pre = 0, task_pre = 0;
-	uint32_t n0 = 0, n1 = 0, total = 0, pool_base = 0, task_o = 0, task_t = 0, task_g = 0;
+	uint32_t n0 = 0, n1 = 0, total = 0, pool_base = 0;
 	// AABB<uint32_t>::Merge (dep/AABB.hpp:52-59) of the first mesh of each pass, per lane, reduced at the end
 	uint32_t box[2][6];
 #pragma unroll
@@ -1056,20 +1066,27 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 				}
 			}
 		}
-		if (wave * 64 < kTasks) {
-			const bool mine = tid < kTasks && (counting || (task_g < w1 && task_g + q_o + q_t > w0));
-			if (counting)
-				greedy_count(s, tid, mine, q_o, q_t);
-			else
-				greedy_emit(s, tid, mine, [&](const Quad &q) {
-					const uint32_t g = task_g + q.ord;
+		if (counting) {
+			if (wave * 64 < kTasks)
+				greedy_count(s, tid, tid < kTasks, q_o, q_t);
+		} else {
+#pragma unroll 1
+			for (int item = tid; item < kTasks * 16; item += NT) {
+				const int t = item >> 4;
+				const uint32_t tg = s.tpre[t];
+				if (tg >= w1 || s.tpre[t + 1] <= w0) // none of the task's quads is in this window
+					continue;
+				const uint32_t to = s.qpre_o[t];
+				describe_row(s, t, item & 15, [&](const Quad &q) {
+					const uint32_t g = tg + q.ord;
 					if (g >= w0 && g < w1) {
-						const uint32_t i = (q.trans ? task_t : task_o) + q.ord_pass;
-						s.list[g - w0] = make_uint2(uint32_t(tid) | (q.u << 7) | (q.v0 << 11) | ((q.w - 1) << 15) |
+						const uint32_t i = q.trans ? (tg - to) + q.ord_pass : to + q.ord_pass;
+						s.list[g - w0] = make_uint2(uint32_t(t) | (q.u << 7) | (q.v0 << 11) | ((q.w - 1) << 15) |
 						                                ((q.h - 1) << 19),
 						                            i | (q.trans << 31));
 					}
 				});
+			}
 		}
 		if (counting) {
 			uint64_t cust_total, task_total;
@@ -1083,8 +1100,14 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 					p.results[job] = hcm_job_result{HCM_OK, 0u, 0u, 0u};
 				return;
 			}
-			task_o = cust0 + (uint32_t)task_pre, task_t = cust1 + (uint32_t)(task_pre >> 32);
-			task_g = cust0 + cust1 + (uint32_t)task_pre + (uint32_t)(task_pre >> 32);
+			// per task: first emission position (custom quads come first) and first opaque index; the
+			// first transparent index is the difference of the two
+			if (tid < kTasks) {
+				s.tpre[tid] = cust0 + cust1 + (uint32_t)task_pre + (uint32_t)(task_pre >> 32);
+				s.qpre_o[tid] = cust0 + (uint32_t)task_pre;
+			}
+			if (tid == 0)
+				s.tpre[kTasks] = total;
 			// 5. allocation
 			if (tid == 0) {
 				const uint32_t nm0 = (n0 + kSplitQuads - 1) / kSplitQuads, nm1 = (n1 + kSplitQuads - 1) / kSplitQuads;
