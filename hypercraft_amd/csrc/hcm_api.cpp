@@ -129,11 +129,21 @@ int build_tables(const hcm_registry *r, DeviceTables *t) {
 			cf |= kCodeEmpty;
 		t->codes.push_back(uint32_t(tex) | cf << 16);
 	}
-	t->pid_of.resize(65536);
+	t->pid_of.resize(65536 + 256);
 	for (uint32_t d = 0; d < 65536; ++d) {
 		if (r->row_of[d] >= r->n_rows)
 			return HCM_ERR_INVALID;
 		t->pid_of[d] = (uint8_t)r->row_of[d];
+	}
+	// per block id: its row when no meta value changes it (kept in LDS by the kernel), else 0xff
+	for (uint32_t id = 0; id < 256; ++id) {
+		uint8_t v = t->pid_of[id];
+		for (uint32_t meta = 1; meta < 256; ++meta)
+			if (t->pid_of[id | meta << 8] != v) {
+				v = 0xff;
+				break;
+			}
+		t->pid_of[65536 + id] = v;
 	}
 	t->meshes.assign(r->meshes, r->meshes + r->n_meshes);
 	t->tex_flags.assign(r->tex_flags, r->tex_flags + r->n_tex);

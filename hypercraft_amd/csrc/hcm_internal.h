@@ -29,7 +29,8 @@ constexpr uint32_t kCodeLiquid = 4u;
 constexpr uint32_t kCodeEmpty = 8u;
 
 struct DevReg {
-	const uint8_t *pid_of;         // [65536] Block::m_data -> texture row (pid)
+	const uint8_t *pid_of;         // [65536] Block::m_data -> texture row (pid); then [256] by block id:
+	                               // the row when the meta byte does not change it, else 0xff
 	const uint2 *rows;             // [n_pid]: x = tc0..tc3 (8 bits each), y = tc4 | tc5 << 8 | mesh << 24
 	const uint32_t *pid_vox;       // [n_pid]: voxel-word template (bits 8..31 above)
 	const uint32_t *codes;         // [n_codes]

@@ -117,7 +117,7 @@ template <int NT> struct Smem {
 	uint32_t pid_vox[kMaxPid];
 	uint32_t codes[kMaxCodes];
 	uint32_t custom_bits[128]; // in-chunk cells with a custom mesh
-	uint32_t flow_bits[(kHaloVox + 31) / 32];
+	uint8_t pid_id[256];       // texture row by block id when the meta byte does not change it, else 0xff
 	uint32_t aabb[2][kMaxMeshPerPass][6];
 	uint32_t events[16][2];
 	uint64_t scan[NT / 64 + 1];
@@ -135,6 +135,12 @@ template <int NT> __device__ __forceinline__ uint32_t row_code(const Smem<NT> &s
 	const uint2 r = s.rows[pid];
 	const uint32_t w = face < 4 ? r.x : r.y;
 	return (w >> (8 * (face & 3))) & 0xffu;
+}
+
+// Texture row of a block value: from LDS for most ids, from the full table for meta-dependent ones.
+template <int NT> __device__ __forceinline__ uint32_t block_pid(const MeshParams &p, const Smem<NT> &s, uint32_t b) {
+	const uint32_t q = s.pid_id[b & 0xffu];
+	return q != 0xffu ? q : p.reg.pid_of[b];
 }
 
 __device__ __forceinline__ uint32_t face_class(uint32_t vox, int face) {
@@ -390,7 +396,7 @@ __device__ __forceinline__ bool tex_trans_pass(const MeshParams &p, uint16_t t) 
 template <int NT> __device__ __forceinline__ CustomCell custom_cell(const MeshParams &p, const Smem<NT> &s, int cell) {
 	const int x = cell & 15, z = (cell >> 4) & 15, y = cell >> 8;
 	const uint16_t b = p.blocks[(size_t)s.slots[26] * HCM_CHUNK_VOXELS + (uint32_t)cell];
-	const uint32_t pid = p.reg.pid_of[b];
+	const uint32_t pid = block_pid(p, s, b);
 	CustomCell c;
 	c.mesh = s.rows[pid].y >> 24;
 	c.flow = (s.pid_vox[pid] & kVoxFlow) ? 1u : 0u;
@@ -797,11 +803,11 @@ __device__ __forceinline__ void greedy_count(Smem<NT> &s, int t, bool active, ui
 				// row u + k, if it exists: every column of a surviving run must continue into it
 				if (!__ballot(alive != 0))
 					break;
-				uint32_t ok = 0;
+				uint32_t ok = 0; // rows past 15 read as empty from the shifted windows
 				if (k < 16) {
 					const uint32_t evr = (ev[(k - 1) >> 1] >> (16 * ((k - 1) & 1))) & 0xffffu;
 					const uint32_t cvr = (cov[k >> 1] >> (16 * (k & 1))) & 0xffffu;
-					ok = u + k < 16 ? (evr & ~cvr) : 0u;
+					ok = evr & ~cvr;
 				}
 				const uint32_t bad = cols & ~ok;
 				if (bad) {
@@ -881,8 +887,6 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 		s.slots[tid] = p.nbr[27 * (size_t)job + tid];
 	for (int i = tid; i < 128; i += NT)
 		s.custom_bits[i] = 0;
-	for (int i = tid; i < (kHaloVox + 31) / 32; i += NT)
-		s.flow_bits[i] = 0;
 	if (tid < 2 * kMaxMeshPerPass * 6) {
 		uint32_t *a = &s.aabb[0][0][0];
 		a[tid] = (tid % 6) < 3 ? 0xffffffffu : 0u; // AABB<uint32_t>{}: min = max(), max = min() (dep/AABB.hpp:11)
@@ -891,8 +895,10 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 		s.n_events = 0, s.flags = 0;
 	__syncthreads();
 
-	// 1. halo staging: one thread per x-row (hy, hz) of 18 voxels
 	const uint16_t ref_block = p.blocks[(size_t)s.slots[26] * HCM_CHUNK_VOXELS];
+	// 1. halo staging: one thread per x-row (hy, hz) of 18 voxels; the lookups of a row are issued as
+	// batches (block id -> texture row in LDS, the rare meta-dependent ids from the full table, then the
+	// rows' voxel-word templates), not as one dependent chain per voxel
 	uint32_t my_flags = 0;
 	for (int row = tid; row < kHaloRows; row += NT) {
 		const int hz = row % kHalo, hy = row / kHalo, y = hy - 1, z = hz - 1;
@@ -914,26 +920,35 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 			lt[x + 1] = (lw[x >> 2] >> (8 * (x & 3))) & 0xffu;
 		}
 		uint32_t pids[18];
+		bool meta_dependent = false;
+#pragma unroll
+		for (int x = 0; x < 18; ++x) {
+			pids[x] = s.pid_id[blk[x] & 0xffu];
+			meta_dependent |= pids[x] == 0xffu;
+		}
+		if (meta_dependent) {
+#pragma unroll
+			for (int x = 0; x < 18; ++x)
+				if (pids[x] == 0xffu)
+					pids[x] = p.reg.pid_of[blk[x]];
+		}
+		uint32_t pv[18];
 #pragma unroll
 		for (int x = 0; x < 18; ++x)
-			pids[x] = p.reg.pid_of[blk[x]];
+			pv[x] = s.pid_vox[pids[x]];
 		const bool inside_yz = (unsigned)y < 16u && (unsigned)z < 16u;
 		uint32_t mesh_bits = 0;
 		uint32_t *dst = s.vox + row * kHalo;
 #pragma unroll
 		for (int x = 0; x < 18; ++x) {
-			const uint32_t pv = s.pid_vox[pids[x]];
-			dst[x] = lt[x] | pv;
+			dst[x] = lt[x] | pv[x];
 			if (blk[x] != ref_block)
 				my_flags |= kFlagNonUniform;
-			if (pv & kVoxMesh) {
+			if (pv[x] & kVoxMesh) {
 				if (inside_yz && x >= 1 && x <= 16)
 					mesh_bits |= 1u << (x - 1);
-				if (pv & kVoxFlow) {
-					const int i = row * kHalo + x;
-					atomicOr(&s.flow_bits[i >> 5], 1u << (i & 31));
+				if (pv[x] & kVoxFlow)
 					my_flags |= kFlagFlow;
-				}
 			}
 		}
 		if (mesh_bits) {
@@ -957,7 +972,7 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 	if (flags & kFlagFlow) {
 		constexpr uint32_t kKeep = 0xffu | kVoxDirect | kVoxIndirect;
 		for (int i = tid; i < kHaloVox; i += NT) {
-			if (!((s.flow_bits[i >> 5] >> (i & 31)) & 1u))
+			if (!(s.vox[i] & kVoxFlow)) // each thread rewrites only its own cells
 				continue;
 			const int hx = i % kHalo, rest = i / kHalo, hz = rest % kHalo, hy = rest / kHalo;
 			const uint32_t pid = s.vox[i] >> kVoxPidShift;
@@ -1233,6 +1248,8 @@ template <int NT> __global__ __launch_bounds__(NT, NT >= 512 ? 2 : 3) void mesh_
 	}
 	for (int i = threadIdx.x; i < kMaxCodes; i += NT)
 		s.codes[i] = i < (int)p.reg.n_codes ? p.reg.codes[i] : (kCodeEmpty | kCodeTransparent) << 16;
+	for (int i = threadIdx.x; i < 256; i += NT)
+		s.pid_id[i] = p.reg.pid_of[65536 + i];
 	if (!p.persistent) {
 		__syncthreads();
 		const uint32_t job = p.job_base + blockIdx.x;
