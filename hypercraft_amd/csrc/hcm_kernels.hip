@@ -113,6 +113,8 @@ template <int NT> struct Smem {
 	};
 	uint32_t tpre[kTasks + 2];  // prefix of visible faces per task; later of quads in emission order
 	uint32_t qpre_o[kTasks];     // per task: index of its first opaque quad in the opaque pass
+	uint32_t rpre[kTasks + 1];   // prefix over tasks of their mask rows holding quad starts
+	uint16_t rmask[kTasks];      // per task: the mask rows holding quad starts
 	uint2 rows[kMaxPid];
 	uint32_t pid_vox[kMaxPid];
 	uint32_t codes[kMaxCodes];
@@ -784,12 +786,14 @@ __device__ __forceinline__ void greedy_count(Smem<NT> &s, int t, bool active, ui
 		ev[k] = active ? evs[k] : 0u;
 	}
 	n_o = 0, n_t = 0;
+	uint32_t rows = 0; // rows holding quad starts
 #pragma unroll 1
 	for (int u = 0; u < 16; ++u) {
 		const uint32_t ne = active ? s.ne[t][u] : 0u, eh = active ? s.eh[t][u] : 0u, tp = active ? s.tp[t][u] : 0u;
 		const uint32_t S = ne & ~(cov[0] & 0xffffu); // uncovered visible cells of row u
 		const uint32_t link = S & ((S & eh) << 1);   // v continues the run of v - 1
 		const uint32_t starts = S & ~link;
+		rows |= (starts ? 1u : 0u) << u;
 		if (active) { // row u's results replace its (already read) inputs; ev's rows are in registers
 			s.ne[t][u] = uint16_t(ne & ~S);     // cells covered by quads of earlier rows
 			s.ev[t][u] = uint16_t(S);
@@ -830,10 +834,34 @@ __device__ __forceinline__ void greedy_count(Smem<NT> &s, int t, bool active, ui
 			ev[k] = (ev[k] >> 16) | (nv << 16);
 		}
 	}
+	if (active) {
+		s.rmask[t] = uint16_t(rows);
+		// the covered cells, transposed to columns (bit r of column v = row r), for the quad heights:
+		// block swaps of 8, 4, 2 and 1 rows and columns
+		uint32_t a[16];
+#pragma unroll
+		for (int r = 0; r < 16; ++r)
+			a[r] = s.ne[t][r];
+#pragma unroll
+		for (int j = 8, m = 0x00ff; j != 0; j >>= 1, m ^= m << j) {
+#pragma unroll
+			for (int k = 0; k < 16; ++k) {
+				if (k & j)
+					continue;
+				const uint32_t x = ((a[k] >> j) ^ a[k + j]) & uint32_t(m);
+				a[k] ^= x << j;
+				a[k + j] ^= x;
+			}
+		}
+#pragma unroll
+		for (int v = 0; v < 16; ++v)
+			s.ne[t][v] = uint16_t(a[v]);
+	}
 }
 
 // The quads of mask row (t, u), from what the counting pass left: C (cells covered by quads of earlier
-// rows) in ne, the uncovered set S in ev, and the opaque and transparent quad starts in eh and tp. A run
+// rows) in ne, transposed to columns, the uncovered set S in ev, and the opaque and transparent quad
+// starts in eh and tp. A run
 // of S continues through every cell that is not a start. A quad from (u, v0) covers rows u .. u + h - 1
 // of its columns, so below it column v0 is in C exactly for rows u + 1 .. u + h - 1: the cell under its
 // last row is either not covered or starts a quad. Quads are numbered within the task in the
@@ -844,16 +872,13 @@ template <int NT, typename Fn> __device__ __forceinline__ void describe_row(cons
 		return;
 	const uint64_t *o64 = reinterpret_cast<const uint64_t *>(s.eh[t]);
 	const uint64_t *t64 = reinterpret_cast<const uint64_t *>(s.tp[t]);
-	const uint64_t *c64 = reinterpret_cast<const uint64_t *>(s.ne[t]);
 	uint32_t n_o = 0, n_t = 0; // quads of each pass in rows 0 .. u - 1
-	uint64_t c[4];
 #pragma unroll
 	for (int q = 0; q < 4; ++q) {
 		const int nb = 16 * (u - 4 * q);
 		const uint64_t m = nb >= 64 ? ~0ull : (nb > 0 ? (1ull << nb) - 1ull : 0ull);
 		n_o += __popcll(o64[q] & m);
 		n_t += __popcll(t64[q] & m);
-		c[q] = c64[q];
 	}
 	const uint32_t link = uint32_t(s.ev[t][u]) & ~starts;
 	uint32_t rest = starts;
@@ -862,12 +887,8 @@ template <int NT, typename Fn> __device__ __forceinline__ void describe_row(cons
 		rest &= rest - 1;
 		const uint32_t below = (1u << v0) - 1u;
 		const uint32_t tr = (st_t >> v0) & 1u;
-		uint32_t h = 1;
-		for (int r = u + 1; r < 16; ++r, ++h) {
-			const uint64_t w = r < 4 ? c[0] : (r < 8 ? c[1] : (r < 12 ? c[2] : c[3]));
-			if (!((w >> (16 * (r & 3) + v0)) & 1ull))
-				break;
-		}
+		// rows u + 1 .. covered in column v0 (the column's bits past row 15 are zero)
+		const uint32_t h = 1u + __builtin_ctz(~(uint32_t(s.ne[t][v0]) >> (u + 1)));
 		Quad q;
 		q.u = u, q.v0 = v0, q.trans = tr, q.h = h;
 		q.w = __ffs(~(link >> (v0 + 1)));
@@ -1085,14 +1106,27 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 			if (wave * 64 < kTasks)
 				greedy_count(s, tid, tid < kTasks, q_o, q_t);
 		} else {
+			const uint32_t n_rows = s.rpre[kTasks];
 #pragma unroll 1
-			for (int item = tid; item < kTasks * 16; item += NT) {
-				const int t = item >> 4;
+			for (uint32_t i = tid; i < n_rows; i += NT) { // the mask rows holding quad starts
+				int lo = 0, hi = kTasks - 1; // task: rpre[t] <= i < rpre[t + 1]
+				while (lo < hi) {
+					const int mid = (lo + hi + 1) >> 1;
+					if (s.rpre[mid] <= i)
+						lo = mid;
+					else
+						hi = mid - 1;
+				}
+				const int t = lo;
 				const uint32_t tg = s.tpre[t];
 				if (tg >= w1 || s.tpre[t + 1] <= w0) // none of the task's quads is in this window
 					continue;
+				uint32_t bits = s.rmask[t], rem = i - s.rpre[t];
+				while (rem--)
+					bits &= bits - 1;
+				const int u = __ffs(bits) - 1;
 				const uint32_t to = s.qpre_o[t];
-				describe_row(s, t, item & 15, [&](const Quad &q) {
+				describe_row(s, t, u, [&](const Quad &q) {
 					const uint32_t g = tg + q.ord;
 					if (g >= w0 && g < w1) {
 						const uint32_t i = q.trans ? (tg - to) + q.ord_pass : to + q.ord_pass;
@@ -1123,6 +1157,24 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 			}
 			if (tid == 0)
 				s.tpre[kTasks] = total;
+			if (wave == 0) { // exclusive prefix over the tasks of their rows holding quad starts
+				const uint32_t a = lane < kTasks ? __popc(s.rmask[lane]) : 0u;
+				const uint32_t b = lane + 64 < kTasks ? __popc(s.rmask[lane + 64]) : 0u;
+				uint32_t ia = a, ib = b;
+#pragma unroll
+				for (int d = 1; d < 64; d <<= 1) {
+					const uint32_t xa = __shfl_up(ia, d), xb = __shfl_up(ib, d);
+					if (lane >= d)
+						ia += xa, ib += xb;
+				}
+				const uint32_t tot_a = __shfl(ia, 63);
+				if (lane < kTasks)
+					s.rpre[lane] = ia - a;
+				if (lane + 64 < kTasks)
+					s.rpre[lane + 64] = tot_a + ib - b;
+				if (lane == 63)
+					s.rpre[kTasks] = tot_a + ib;
+			}
 			// 5. allocation
 			if (tid == 0) {
 				const uint32_t nm0 = (n0 + kSplitQuads - 1) / kSplitQuads, nm1 = (n1 + kSplitQuads - 1) / kSplitQuads;
