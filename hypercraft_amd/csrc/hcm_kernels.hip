@@ -644,7 +644,8 @@ __device__ __forceinline__ int face_owner(int t, int u, int v, int &face) {
 // Light4 (light4_init, :48-163) in x and the low 24 bits of y, the texture code in y bits 24-30 and the
 // transparent-pass flag in bit 31. Faces merge only if their words are equal (Light4::operator==, :44,
 // and the texture compare, :367-368).
-template <int NT> __device__ __forceinline__ uint2 face_word(const Smem<NT> &s, int t0, int t1, uint32_t gi) {
+template <int NT>
+__device__ __forceinline__ uint2 face_word(const Smem<NT> &s, int t0, int t1, uint32_t gi, uint32_t &desc) {
 	int lo = t0, hi = t1 - 1; // the task holding gi: tpre[t] <= gi < tpre[t + 1]
 	while (lo < hi) {
 		const int mid = (lo + hi + 1) >> 1;
@@ -689,6 +690,7 @@ template <int NT> __device__ __forceinline__ uint2 face_word(const Smem<NT> &s, 
 		rem -= c2, bits >>= 2, v += 2;
 	if (rem >= (bits & 1u))
 		v += 1;
+	desc = uint32_t(t) | uint32_t(u) << 7 | uint32_t(v) << 11;
 	int face;
 	const int self = face_owner(t, u, v, face);
 	const uint2 l = light4(s.vox, face, self);
@@ -697,42 +699,27 @@ template <int NT> __device__ __forceinline__ uint2 face_word(const Smem<NT> &s, 
 	return make_uint2(l.x, l.y | (tc << 24) | (tpass << 31));
 }
 
-// Equal-to-right, equal-to-below and transparent-pass masks of mask row (t, u) from the face words of
-// rows u and u + 1 (consecutive in the dense order), without atomics.
-template <int NT> __device__ __forceinline__ void row_links(Smem<NT> &s, int t, int u, uint32_t g0) {
-	const uint32_t ne_u = s.ne[t][u];
-	uint32_t eh = 0, ev = 0, tp = 0;
-	if (ne_u) {
-		const uint64_t *rows = reinterpret_cast<const uint64_t *>(s.ne[t]);
-		uint32_t above = 0; // faces in rows 0 .. u - 1
-#pragma unroll
-		for (int q = 0; q < 4; ++q) {
-			const uint64_t w = rows[q];
-			const int nb = 16 * (u - 4 * q); // bits of this word in rows above u
-			above += nb >= 64 ? __popcll(w) : (nb > 0 ? __popcll(w & ((1ull << nb) - 1ull)) : 0u);
-		}
-		const uint32_t i0 = s.tpre[t] - g0 + above, i1 = i0 + __popc(ne_u);
-		const uint32_t ne_d = u < 15 ? s.ne[t][u + 1] : 0u;
-		uint32_t bits = ne_u, k = 0;
-		while (bits) {
-			const uint32_t v = __ffs(bits) - 1;
-			bits &= bits - 1;
-			const uint2 w = s.key[i0 + k];
-			tp |= (w.y >> 31) << v;
-			if (v < 15 && ((ne_u >> (v + 1)) & 1u)) {
-				const uint2 r = s.key[i0 + k + 1];
-				eh |= uint32_t(r.x == w.x && r.y == w.y) << v;
-			}
-			if ((ne_d >> v) & 1u) {
-				const uint2 d = s.key[i1 + __popc(ne_d & ((1u << v) - 1u))];
-				ev |= uint32_t(d.x == w.x && d.y == w.y) << v;
-			}
-			++k;
-		}
+// Equal-to-right, equal-to-below and transparent-pass bits of the visible face with dense index i (local
+// to the group; desc = t | u << 7 | v << 11): its right and lower neighbours, when visible, are at
+// i + 1 and after the rest of row u and the start of row u + 1 in the dense order.
+template <int NT> __device__ __forceinline__ void face_links(Smem<NT> &s, uint32_t i, uint32_t desc) {
+	const int t = desc & 127, u = (desc >> 7) & 15, v = (desc >> 11) & 15;
+	const uint32_t ne_u = s.ne[t][u], ne_d = u < 15 ? s.ne[t][u + 1] : 0u;
+	const uint2 w = s.key[i];
+	uint32_t *row = reinterpret_cast<uint32_t *>(&s.eh[t][u & ~1]);
+	const uint32_t bit = 1u << (v + 16 * (u & 1));
+	if (v < 15 && ((ne_u >> (v + 1)) & 1u)) {
+		const uint2 r = s.key[i + 1];
+		if (r.x == w.x && r.y == w.y)
+			atomicOr(row, bit);
 	}
-	s.eh[t][u] = uint16_t(eh);
-	s.ev[t][u] = uint16_t(ev);
-	s.tp[t][u] = uint16_t(tp);
+	if ((ne_d >> v) & 1u) {
+		const uint2 d = s.key[i + 1 + __popc(ne_u >> (v + 1)) + __popc(ne_d & ((1u << v) - 1u))];
+		if (d.x == w.x && d.y == w.y)
+			atomicOr(reinterpret_cast<uint32_t *>(&s.ev[t][u & ~1]), bit);
+	}
+	if (w.y >> 31)
+		atomicOr(reinterpret_cast<uint32_t *>(&s.tp[t][u & ~1]), bit);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -1054,11 +1041,37 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 		__syncthreads();
 		const int t1 = s.group_end;
 		const uint32_t g0 = s.tpre[t0], n = s.tpre[t1] - g0;
-		for (uint32_t i = tid; i < n; i += NT)
-			s.key[i] = face_word(s, t0, t1, g0 + i);
+		for (int k = tid; k < (t1 - t0) * 8; k += NT) { // clear the group's link masks
+			const int t = t0 + (k >> 3), q = k & 7;
+			reinterpret_cast<uint32_t *>(s.eh[t])[q] = 0;
+			reinterpret_cast<uint32_t *>(s.ev[t])[q] = 0;
+			reinterpret_cast<uint32_t *>(s.tp[t])[q] = 0;
+		}
+		// a lane keeps the positions of its faces (at most kListCap / NT of them) for the links, in a
+		// register queue shifted once per step (rolled loops, static register indices)
+		constexpr int kMine = (kListCap + NT - 1) / NT;
+		uint32_t mine[kMine] = {};
+#pragma unroll 1
+		for (int k = 0; k < kMine; ++k) {
+			const uint32_t i = tid + k * NT;
+			uint32_t d = 0;
+			if (i < n)
+				s.key[i] = face_word(s, t0, t1, g0 + i, d);
+#pragma unroll
+			for (int j = 0; j + 1 < kMine; ++j)
+				mine[j] = mine[j + 1];
+			mine[kMine - 1] = d;
+		}
 		__syncthreads();
-		for (int item = tid; item < (t1 - t0) * 16; item += NT)
-			row_links(s, t0 + (item >> 4), item & 15, g0);
+#pragma unroll 1
+		for (int k = 0; k < kMine; ++k) {
+			const uint32_t i = tid + k * NT;
+			if (i < n)
+				face_links(s, i, mine[0]);
+#pragma unroll
+			for (int j = 0; j + 1 < kMine; ++j)
+				mine[j] = mine[j + 1];
+		}
 		__syncthreads();
 		t0 = t1;
 	}
