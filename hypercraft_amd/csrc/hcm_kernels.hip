@@ -125,6 +125,7 @@ template <int NT> struct Smem {
 	uint64_t scan[NT / 64 + 1];
 	uint32_t slots[27];
 	uint32_t job, flags, n_events, status;
+	uint32_t wave_ref[NT / 64];
 	uint32_t pool_base, mesh_base, n_mesh0, n_mesh1, group_end;
 };
 
@@ -903,11 +904,10 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 		s.n_events = 0, s.flags = 0;
 	__syncthreads();
 
-	const uint16_t ref_block = p.blocks[(size_t)s.slots[26] * HCM_CHUNK_VOXELS];
 	// 1. halo staging: one thread per x-row (hy, hz) of 18 voxels; the lookups of a row are issued as
 	// batches (block id -> texture row in LDS, the rare meta-dependent ids from the full table, then the
 	// rows' voxel-word templates), not as one dependent chain per voxel
-	uint32_t my_flags = 0;
+	uint32_t my_flags = 0, wave_ref = 0xffffffffu;
 	for (int row = tid; row < kHaloRows; row += NT) {
 		const int hz = row % kHalo, hy = row / kHalo, y = hy - 1, z = hz - 1;
 		const size_t lrow = 16u * (uint32_t(z & 15) + 16u * uint32_t(y & 15));
@@ -927,6 +927,11 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 			blk[x + 1] = (bw[x >> 1] >> (16 * (x & 1))) & 0xffffu;
 			lt[x + 1] = (lw[x >> 2] >> (8 * (x & 3))) & 0xffu;
 		}
+		// one reference value per wavefront (its first row's first voxel): a halo of one block value has
+		// every voxel equal to it and all wavefronts' references equal
+		const uint32_t v0 = __builtin_amdgcn_readfirstlane(blk[0]);
+		if (row < NT)
+			wave_ref = v0;
 		uint32_t pids[18];
 		bool meta_dependent = false;
 #pragma unroll
@@ -950,7 +955,7 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 #pragma unroll
 		for (int x = 0; x < 18; ++x) {
 			dst[x] = lt[x] | pv[x];
-			if (blk[x] != ref_block)
+			if (blk[x] != wave_ref)
 				my_flags |= kFlagNonUniform;
 			if (pv[x] & kVoxMesh) {
 				if (inside_yz && x >= 1 && x <= 16)
@@ -967,8 +972,15 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 	}
 	if (my_flags)
 		atomicOr(&s.flags, my_flags);
+	if (lane == 0)
+		s.wave_ref[wave] = wave_ref; // 0xffffffff: the wavefront staged no row
 	__syncthreads();
-	const uint32_t flags = s.flags;
+	uint32_t flags = s.flags;
+#pragma unroll
+	for (int w = 1; w < NT / 64; ++w)
+		if (s.wave_ref[w] != 0xffffffffu && s.wave_ref[w] != s.wave_ref[0])
+			flags |= kFlagNonUniform;
+
 	if (!(flags & kFlagNonUniform) && !(flags & kFlagCustom)) {
 		// one block value everywhere: every face is hidden by an equal neighbour (BlockTexture::Show)
 		if (tid == 0)
