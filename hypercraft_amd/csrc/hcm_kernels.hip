@@ -222,6 +222,21 @@ __device__ __forceinline__ bool l4_flip(uint2 l) {
 
 __device__ __forceinline__ uint64_t ballot64(bool b) { return __ballot(b); }
 
+// Minimum / maximum over the wavefront with DPP lane moves (no LDS): swaps within quads, the half-row
+// and row mirrors leave every lane of a 16-lane row with the row's result, then the row broadcasts of
+// lanes 15 and 31 fold the rows into lane 63. Lanes a move does not write keep the identity.
+template <bool kMax> __device__ __forceinline__ uint32_t wave_fold(uint32_t v) {
+	constexpr uint32_t id = kMax ? 0u : 0xffffffffu;
+	auto op = [](uint32_t a, uint32_t b) { return kMax ? max(a, b) : min(a, b); };
+	v = op(v, (uint32_t)__builtin_amdgcn_update_dpp((int)id, (int)v, 0xb1, 0xf, 0xf, false));  // quad_perm 1,0,3,2
+	v = op(v, (uint32_t)__builtin_amdgcn_update_dpp((int)id, (int)v, 0x4e, 0xf, 0xf, false));  // quad_perm 2,3,0,1
+	v = op(v, (uint32_t)__builtin_amdgcn_update_dpp((int)id, (int)v, 0x141, 0xf, 0xf, false)); // row_half_mirror
+	v = op(v, (uint32_t)__builtin_amdgcn_update_dpp((int)id, (int)v, 0x140, 0xf, 0xf, false)); // row_mirror
+	v = op(v, (uint32_t)__builtin_amdgcn_update_dpp((int)id, (int)v, 0x142, 0xa, 0xf, false)); // row_bcast:15
+	v = op(v, (uint32_t)__builtin_amdgcn_update_dpp((int)id, (int)v, 0x143, 0xc, 0xf, false)); // row_bcast:31
+	return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
 // ------------------------------------------------------------------------------------------------
 // block-wide exclusive scan of 64-bit values (two 32-bit counters packed lo | hi << 32)
 // ------------------------------------------------------------------------------------------------
@@ -1267,15 +1282,8 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 #pragma unroll
 	for (int pass = 0; pass < 2; ++pass)
 #pragma unroll
-		for (int a = 0; a < 6; ++a) {
-			uint32_t b = box[pass][a];
-#pragma unroll
-			for (int d = 1; d < 64; d <<= 1) {
-				const uint32_t o = __shfl_xor(b, d);
-				b = a < 3 ? min(b, o) : max(b, o);
-			}
-			box[pass][a] = b;
-		}
+		for (int a = 0; a < 6; ++a)
+			box[pass][a] = a < 3 ? wave_fold<false>(box[pass][a]) : wave_fold<true>(box[pass][a]);
 	if (lane == 0 && (n0 | n1)) {
 		if (n0)
 			aabb_merge(s, 0, 0, box[0], box[0] + 3);
