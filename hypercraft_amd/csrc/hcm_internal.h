@@ -40,11 +40,12 @@ struct DevReg {
 	uint32_t flow_tex, src_id, flow_id, pad;
 };
 
+// Each counter on its own 256-byte line: the three are hit by device-scope atomics from every
+// workgroup, and atomics on one line are serialised where that line lives.
 struct Counters {
-	uint32_t queue;       // persistent work-queue head
-	uint32_t pool_cursor; // quads allocated in the pool
-	uint32_t mesh_cursor; // mesh records allocated
-	uint32_t pad;
+	alignas(256) uint32_t queue;       // persistent work-queue head
+	alignas(256) uint32_t pool_cursor; // quads allocated in the pool
+	alignas(256) uint32_t mesh_cursor; // mesh records allocated
 };
 
 struct MeshParams {

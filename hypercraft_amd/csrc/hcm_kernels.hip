@@ -1359,7 +1359,6 @@ __global__ void reset_kernel(Counters *ctr) {
 		ctr->queue = 0;
 		ctr->pool_cursor = 0;
 		ctr->mesh_cursor = 0;
-		ctr->pad = 0;
 	}
 }
 
