@@ -40,10 +40,18 @@ struct DevReg {
 	uint32_t flow_tex, src_id, flow_id, pad;
 };
 
-// Each counter on its own 256-byte line: the three are hit by device-scope atomics from every
-// workgroup, and atomics on one line are serialised where that line lives.
+// The persistent kernel's job list is cut into kQueues contiguous parts, one per XCD (gfx950 has 8,
+// and workgroups are dispatched to them round-robin by workgroup id). A workgroup takes jobs from its
+// own XCD's part first, so neighbouring chunks, which share halo data, are read through one L2; then it
+// takes jobs from the other parts. Each counter sits on its own 256-byte line: all are hit by
+// device-scope atomics, and atomics on one line are serialised where that line lives.
+constexpr uint32_t kQueues = 8;
+
 struct Counters {
-	alignas(256) uint32_t queue;       // persistent work-queue head
+	struct alignas(256) Line {
+		uint32_t v;
+	};
+	Line queue[kQueues];               // jobs taken from each part of the job list
 	alignas(256) uint32_t pool_cursor; // quads allocated in the pool
 	alignas(256) uint32_t mesh_cursor; // mesh records allocated
 };

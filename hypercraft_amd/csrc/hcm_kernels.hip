@@ -1,18 +1,21 @@
 // hcm_kernels.hip -- gfx950 chunk-meshing kernels.
 //
-// One workgroup meshes one 16^3 chunk at a time; a persistent workgroup takes chunk jobs from a work
-// queue until it is empty. Per chunk:
+// One workgroup meshes one 16^3 chunk at a time; a persistent workgroup takes chunk jobs from work
+// queues (one per XCD, then the others') until they are empty. Per chunk:
 //   1. staging: one thread per 18-voxel x-row of the 18^3 halo loads the row from its chunk slots with
 //      vector loads and writes one LDS word per voxel: light byte, direct/indirect light-pass bits,
-//      a 2-bit face class per face (empty / opaque / transparent / liquid) and the texture row;
-//      bitmaps mark cells with custom meshes and liquid-flow cells. A chunk whose halo holds a single
-//      block value and no custom mesh has no faces and stops here.
+//      a 2-bit face class per face (empty / opaque / transparent / liquid), custom-mesh and flow bits
+//      and the texture row. A chunk whose halo holds a single block value and no custom mesh has no
+//      faces and stops here.
 //   2. liquid-flow cells take their per-cell face textures (the reference's m_dynamic_block_textures).
-//   3. one wavefront per (axis, slice, direction) task walks the 16x16 mask in groups of 4 rows:
-//      BlockTexture::Show on the class bits gives the visible faces (texture codes are compared only
-//      for transparent against transparent), light4_init gives each visible face's word, and lane
-//      shuffles compare words with the right and lower neighbours; ballots write the visible,
-//      equal-to-right, equal-to-below and transparent-pass masks.
+//   3. faces, per (axis, slice, direction) task, each a 16x16 mask:
+//      - visibility, bit-sliced: one lane per column of an axis keeps its cells' face classes as bit
+//        planes, so BlockTexture::Show is a few bit operations (texture codes are compared only for
+//        transparent against transparent); ballots transpose the columns into the tasks' mask rows;
+//      - face words: visible faces are numbered densely in the reference's scan order and one lane per
+//        face evaluates light4_init; the face word is Light4 plus texture code and pass;
+//      - link masks: each face is compared with its right and lower neighbours, giving the
+//        equal-to-right, equal-to-below and transparent-pass masks.
 //   4. greedy merge: one lane per task replays the reference's merge row by row on the bit masks:
 //      the quads starting in a row are the row's uncovered visible runs split where the face word
 //      changes; each extends downward while every column of it continues (equal and uncovered).
@@ -1342,9 +1345,25 @@ template <int NT> __global__ __launch_bounds__(NT, NT >= 512 ? 2 : 3) void mesh_
 			mesh_chunk(p, s, job);
 		return;
 	}
+	// part q of the job list is [q * n / kQueues, (q + 1) * n / kQueues); a part is empty once its
+	// counter reaches its length, and the workgroup moves on to the next part
+	const uint32_t home = blockIdx.x % kQueues;
+	uint32_t tried = 0;
 	for (;;) {
-		if (threadIdx.x == 0)
-			s.job = atomicAdd(&p.ctr->queue, 1u);
+		if (threadIdx.x == 0) {
+			uint32_t job = ~0u;
+			while (job == ~0u && tried < kQueues) {
+				const uint32_t q = (home + tried) % kQueues;
+				const uint32_t lo = uint32_t(uint64_t(q) * p.n_jobs / kQueues);
+				const uint32_t len = uint32_t(uint64_t(q + 1) * p.n_jobs / kQueues) - lo;
+				const uint32_t k = atomicAdd(&p.ctr->queue[q].v, 1u);
+				if (k < len)
+					job = lo + k;
+				else
+					++tried;
+			}
+			s.job = job;
+		}
 		__syncthreads();
 		const uint32_t job = s.job;
 		if (job >= p.n_jobs)
@@ -1356,7 +1375,8 @@ template <int NT> __global__ __launch_bounds__(NT, NT >= 512 ? 2 : 3) void mesh_
 
 __global__ void reset_kernel(Counters *ctr) {
 	if (threadIdx.x == 0) {
-		ctr->queue = 0;
+		for (uint32_t q = 0; q < kQueues; ++q)
+			ctr->queue[q].v = 0;
 		ctr->pool_cursor = 0;
 		ctr->mesh_cursor = 0;
 	}
