@@ -599,54 +599,68 @@ __device__ __forceinline__ void emit_custom(const MeshParams &p, Smem<NT> &s, ui
 }
 
 // ------------------------------------------------------------------------------------------------
-// visibility by bit slices: one lane per (u, v) column along the slicing axis
+// visibility: one lane per (u, v) column along the slicing axis
 // ------------------------------------------------------------------------------------------------
-// The lane reads the column's 18 halo voxels and keeps the 2-bit face classes of its + and - faces as
-// bit planes over the column (bit a = halo position a). The face between halo positions a and a + 1 is
-// in slice a; BlockTexture::Show (texture/include/texture/BlockTexture.hpp:68-76) on classes is
+// BlockTexture::Show (texture/include/texture/BlockTexture.hpp:68-76) on the 2-bit face classes
+// (empty 0, opaque 1, transparent 2, liquid 3):
 //   opaque self:      visible unless the other face is opaque,
 //   transparent self: visible if the other is empty or liquid, or transparent with another texture,
-//   liquid self:      visible if the other is empty,
-// and only blocks inside the chunk own faces (:330, :335). Ballots over the wave transpose the columns'
-// bits into the 16x16 masks of the (axis, slice, direction) tasks, four mask rows per wave.
-template <int NT> __device__ __forceinline__ void visibility_column(Smem<NT> &s, int A, int col) {
+//   liquid self:      visible if the other is empty.
+// The face between halo positions a and a + 1 is in slice a; with i = class(+ face of a) |
+// class(- face of a + 1) << 2, bit i of kShowPlus says whether the + face of a shows and bit i of
+// kShowMinus whether the - face of a + 1 does. Transparent against transparent (i = 10) depends on
+// the two texture codes and is resolved from the voxels' texture rows.
+__host__ __device__ constexpr uint32_t show_class(uint32_t self, uint32_t other) {
+	return self == 1 ? (other != 1 ? 1u : 0u) : self == 2 ? ((other == 0 || other == 3) ? 1u : 0u) : self == 3 ? (other == 0 ? 1u : 0u) : 0u;
+}
+__host__ __device__ constexpr uint32_t show_table(bool plus) {
+	uint32_t t = 0;
+	for (uint32_t i = 0; i < 16; ++i)
+		t |= (plus ? show_class(i & 3u, i >> 2) : show_class(i >> 2, i & 3u)) << i;
+	return t;
+}
+constexpr uint32_t kShowPlus = show_table(true), kShowMinus = show_table(false), kShowTT = 2u | 2u << 2;
+static_assert(kShowPlus == 0x620eu && kShowMinus == 0x19d0u, "Show on face classes");
+
+// Only blocks inside the chunk own faces (:330, :335): + faces of halo positions 1..16 (slices 1..16)
+// and - faces of positions 1..16 (slices 0..15). Ballots over the wave give the 16x16 masks of the
+// (axis, slice, direction) tasks, four mask rows per wave.
+template <int A, int NT> __device__ __forceinline__ void visibility_column(Smem<NT> &s, int col) {
 	const int lane = threadIdx.x & 63;
 	const int u = col >> 4, v = col & 15;
-	const int sa = axis_stride(A);
-	const int base = (u + 1) * axis_stride(u_axis(A)) + (v + 1) * axis_stride(v_axis(A));
-	const int shp = kVoxClassShift + 4 * A; // class of face 2A (+); face 2A + 1 (-) follows
-	uint32_t p0 = 0, p1 = 0, m0 = 0, m1 = 0;
+	constexpr int sa = axis_stride(A);
+	constexpr int shp = kVoxClassShift + 4 * A; // class of face 2A (+); face 2A + 1 (-) follows
+	const uint32_t *cw = s.vox + (u + 1) * axis_stride(u_axis(A)) + (v + 1) * axis_stride(v_axis(A));
+	uint32_t c[kHalo];
 #pragma unroll
-	for (int a = 0; a < kHalo; ++a) {
-		const uint32_t w = s.vox[base + a * sa] >> shp;
-		p0 |= (w & 1u) << a;
-		p1 |= ((w >> 1) & 1u) << a;
-		m0 |= ((w >> 2) & 1u) << a;
-		m1 |= ((w >> 3) & 1u) << a;
-	}
-	// class bit 0 = opaque or liquid, bit 1 = transparent or liquid
-	const uint32_t q0 = m0 >> 1, q1 = m1 >> 1; // the - face of the voxel at a + 1, at bit a
-	const uint32_t Ep = ~p0 & ~p1, Op = p0 & ~p1, Tp = ~p0 & p1, Lp = p0 & p1;
-	const uint32_t Eq = ~q0 & ~q1, Oq = q0 & ~q1, Tq = ~q0 & q1, Lq = q0 & q1;
-	uint32_t vp = ((Op & ~Oq) | (Tp & (Eq | Lq)) | (Lp & Eq)) & 0x1fffeu; // + face of a, owner a in [1, 16]
-	uint32_t vm = ((Oq & ~Op) | (Tq & (Ep | Lp)) | (Lq & Ep)) & 0xffffu;  // - face of a + 1, owner in [1, 16]
-	// transparent against transparent: visible iff the two textures differ (the same pair of codes for
-	// both directions)
-	uint32_t tt = (Tp & Tq) & 0x1ffffu;
-	while (tt) {
-		const int a = __ffs(tt) - 1;
-		tt &= tt - 1;
-		const uint32_t wa = s.vox[base + a * sa], wb = s.vox[base + (a + 1) * sa];
-		if (row_code(s, wa >> kVoxPidShift, 2 * A) != row_code(s, wb >> kVoxPidShift, 2 * A + 1))
-			vp |= (1u << a) & 0x1fffeu, vm |= (1u << a) & 0xffffu;
-	}
+	for (int a = 0; a < kHalo; ++a)
+		c[a] = (cw[a * sa] >> shp) & 15u;
 	const int q = (col >> 6) & 3; // this wave's group of four mask rows
+	uint64_t tt = 0;              // lanes with transparent against transparent, over all slices
 #pragma unroll
 	for (int a = 0; a <= 16; ++a) {
-		const uint64_t bp = ballot64((vp >> a) & 1u), bm = ballot64((vm >> a) & 1u);
+		const uint32_t i = (c[a] & 3u) | (c[a + 1] & 12u);
+		const uint64_t bp = a >= 1 ? ballot64((kShowPlus >> i) & 1u) : 0ull;
+		const uint64_t bm = a <= 15 ? ballot64((kShowMinus >> i) & 1u) : 0ull;
+		tt |= ballot64(i == kShowTT);
 		if (lane == 0) {
 			reinterpret_cast<uint64_t *>(s.ne[34 * A + 2 * a])[q] = bp;
 			reinterpret_cast<uint64_t *>(s.ne[34 * A + 2 * a + 1])[q] = bm;
+		}
+	}
+	if (tt) { // such faces show when the two texture codes differ (both directions alike)
+#pragma unroll 1
+		for (int a = 0; a <= 16; ++a) {
+			const uint32_t wa = cw[a * sa], wb = cw[(a + 1) * sa];
+			const bool d = (((wa >> shp) & 3u) | ((wb >> shp) & 12u)) == kShowTT &&
+			               row_code(s, wa >> kVoxPidShift, 2 * A) != row_code(s, wb >> kVoxPidShift, 2 * A + 1);
+			const uint64_t bd = ballot64(d);
+			if (bd && lane == 0) {
+				if (a >= 1)
+					reinterpret_cast<uint64_t *>(s.ne[34 * A + 2 * a])[q] |= bd;
+				if (a <= 15)
+					reinterpret_cast<uint64_t *>(s.ne[34 * A + 2 * a + 1])[q] |= bd;
+			}
 		}
 	}
 }
@@ -1024,10 +1038,12 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 
 	// 3. faces: visibility masks by bit slices; then, for each group of tasks whose visible faces fit the
 	// stage, the face words (one lane per face, in dense order) and the link masks (one lane per row)
-#pragma unroll 1
-	for (int A = 0; A < 3; ++A)
-		for (int col = tid; col < 256; col += NT)
-			visibility_column(s, A, col);
+	for (int col = tid; col < 256; col += NT)
+		visibility_column<0>(s, col);
+	for (int col = tid; col < 256; col += NT)
+		visibility_column<1>(s, col);
+	for (int col = tid; col < 256; col += NT)
+		visibility_column<2>(s, col);
 	__syncthreads();
 	if (wave == 0) { // exclusive prefix of visible faces over the tasks
 		uint32_t a = 0, b = 0;
