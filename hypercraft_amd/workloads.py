@@ -189,3 +189,51 @@ def slice_jobs(blocks, light, nbr, jobs):
     """The 27 host chunks of one job (for the plain path and the CPU oracle)."""
     slots = nbr[jobs]
     return [blocks[s] for s in slots], [light[s] for s in slots]
+
+
+# ---- inputs of the light rebuild (sunlight heights per chunk, torchlight sources) ----
+LIGHT_SEED = 6
+GLOWSTONE = 0x0018  # the reference's one light-emitting block (block/block/glowstone.hpp: light level 15)
+
+
+def sunlight_heights(blocks, dims):
+    """Per chunk of a grid store (slot x + dx * (z + dz * y)) its 16 x 16 sunlight heights (InnerPos1, index
+    x + 16 z, Chunk.hpp:65-76): one above the highest non-air voxel of the world column, in chunk-local
+    units clamped to [0, 16] as the terrain generator clamps its light map (DefaultTerrain.cpp:148-152)."""
+    dx, dy, dz = dims
+    solid = (blocks.reshape(dy, dz, dx, 16, 16, 16) & 0xFF) != 0  # [cy, cz, cx, ly, lz, lx]
+    col = solid.transpose(1, 2, 4, 5, 0, 3).reshape(dz, dx, 16, 16, dy * 16)  # [cz, cx, lz, lx, world y]
+    top = np.where(col.any(-1), dy * 16 - np.argmax(col[..., ::-1], -1), 0)  # first sunlit world y
+    cy = np.arange(dy).reshape(dy, 1, 1, 1, 1)
+    h = np.clip(top[None] - 16 * cy, 0, 16).astype(np.int8)  # [cy, cz, cx, lz, lx]
+    return np.ascontiguousarray(h.reshape(dy * dz * dx, 256))
+
+
+def place_glowstone(blocks, seed, fraction):
+    """Turns a seeded fraction of the air voxels into glowstone (torchlight level 15 sources)."""
+    rng = np.random.default_rng(seed)
+    out = blocks.copy()
+    air = np.flatnonzero((out.reshape(-1) & 0xFF) == 0)
+    pick = rng.choice(air, size=int(len(air) * fraction), replace=False) if len(air) else air
+    out.reshape(-1)[pick] = GLOWSTONE
+    return out
+
+
+def lit_cave_region(seed=LIGHT_SEED, n=16, glow=0.0005):
+    """The cave region with glowstone in a fraction of its air and sunlight heights from its columns:
+    (blocks, heights, nbr) for the light rebuild before meshing (ChunkMeshTask.cpp:48-111)."""
+    blocks, _, nbr = cave_region(n=n)
+    blocks = place_glowstone(blocks, seed, glow)
+    return blocks, sunlight_heights(blocks, (n + 2, n + 2, n + 2)), nbr
+
+
+def lit_terrain_chunk(seed=LIGHT_SEED, glow=0.002):
+    """The single terrain chunk neighbourhood with glowstone and sunlight heights (one job)."""
+    blocks, _, nbr = terrain_chunk()
+    blocks = place_glowstone(blocks, seed, glow)
+    return blocks, sunlight_heights(blocks, (3, 3, 3)), nbr
+
+
+def slice_heights(heights, nbr, job):
+    """The 27 sunlight-height arrays of one job."""
+    return [heights[s] for s in nbr[job]]

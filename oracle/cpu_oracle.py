@@ -107,6 +107,11 @@ class Oracle(_Base):
                                             ctypes.POINTER(MeshOut)]
         self.lib.orc_mesh_many.argtypes = [ctypes.POINTER(Registry), ctypes.c_void_p, ctypes.c_void_p,
                                            ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p]
+        self.lib.orc_light_rebuild.argtypes = [ctypes.POINTER(Registry), ctypes.c_void_p, ctypes.c_void_p,
+                                               ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        self.lib.orc_relight_many.argtypes = [ctypes.POINTER(Registry), ctypes.c_void_p, ctypes.c_void_p,
+                                              ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
+                                              ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
         self.reg = Registry()
         r = self.lib.orc_registry_load(registry_path.encode(), ctypes.byref(self.reg))
         if r != 0:
@@ -120,6 +125,38 @@ class Oracle(_Base):
         if r != 0:
             raise RuntimeError(f"orc_mesh_chunk failed: {r}")
         return self.out.meshes()
+
+    def light_rebuild(self, light_level, blocks27, heights27):
+        """The mesh task's light rebuild: -> (window (46, 46, 46) u8 indexed [y, z, x] over [-15, 31),
+        halo (18, 18, 18) u8 indexed [y, z, x] over [-1, 17))."""
+        b = [np.ascontiguousarray(x, dtype=np.uint16).reshape(-1) for x in blocks27]
+        h = [np.ascontiguousarray(x, dtype=np.int8).reshape(-1) for x in heights27]
+        assert len(b) == 27 and len(h) == 27 and all(x.size == VOX for x in b) and all(x.size == 256 for x in h)
+        lv = np.ascontiguousarray(light_level, np.uint8)
+        assert lv.size == 65536
+        win = np.zeros(46 ** 3, np.uint8)
+        halo = np.zeros(18 ** 3, np.uint8)
+        r = self.lib.orc_light_rebuild(ctypes.byref(self.reg), lv.ctypes.data, _ptr_array(b, ctypes.c_uint16),
+                                       _ptr_array(h, ctypes.c_int8), win.ctypes.data, halo.ctypes.data)
+        if r != 0:
+            raise RuntimeError(f"orc_light_rebuild failed: {r}")
+        return win.reshape(46, 46, 46), halo.reshape(18, 18, 18)
+
+    def relight_many(self, light_level, blocks, heights, nbr, n_threads, mesh=True, halo=False):
+        """Light rebuild (and meshing) of every job: -> (halo (J, 18, 18, 18) u8 or None, quads (J,) or None)."""
+        blocks = np.ascontiguousarray(blocks, np.uint16)
+        heights = np.ascontiguousarray(heights, np.int8)
+        nbr = np.ascontiguousarray(nbr, np.uint32)
+        lv = np.ascontiguousarray(light_level, np.uint8)
+        n = nbr.shape[0]
+        h = np.zeros((n, 18, 18, 18), np.uint8) if halo else None
+        q = np.zeros(n, np.uint64) if mesh else None
+        r = self.lib.orc_relight_many(ctypes.byref(self.reg), lv.ctypes.data, blocks.ctypes.data, heights.ctypes.data,
+                                      nbr.ctypes.data, n, n_threads, 1 if mesh else 0,
+                                      h.ctypes.data if halo else None, q.ctypes.data if mesh else None)
+        if r != 0:
+            raise RuntimeError(f"orc_relight_many failed: {r}")
+        return h, q
 
     def mesh_many(self, blocks, light, nbr, n_threads):
         """blocks (S, 4096) u16, light (S, 4096) u8, nbr (J, 27) u32 -> per-job quad counts."""
@@ -143,6 +180,8 @@ class Reference(_Base):
         self.lib.hcref_mesh_many.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
                                              ctypes.c_uint32, ctypes.c_void_p]
         self.lib.hcref_export_registry.argtypes = [ctypes.c_char_p]
+        self.lib.hcref_export_light_levels.argtypes = [ctypes.c_char_p]
+        self.lib.hcref_light_rebuild.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         self.out = _Out()
 
     @staticmethod
@@ -153,6 +192,26 @@ class Reference(_Base):
         r = self.lib.hcref_export_registry(path.encode())
         if r != 0:
             raise RuntimeError(f"hcref_export_registry failed: {r}")
+
+    def light_levels(self):
+        """Block::GetLightLevel for every Block::m_data value, u8[65536]."""
+        import tempfile
+        with tempfile.NamedTemporaryFile(suffix=".bin") as f:
+            r = self.lib.hcref_export_light_levels(f.name.encode())
+            if r != 0:
+                raise RuntimeError(f"hcref_export_light_levels failed: {r}")
+            return np.fromfile(f.name, np.uint8)
+
+    def light_rebuild(self, blocks27, heights27):
+        """The reference's light rebuild -> window (46, 46, 46) u8 indexed [y, z, x] over [-15, 31)."""
+        b = [np.ascontiguousarray(x, dtype=np.uint16).reshape(-1) for x in blocks27]
+        h = [np.ascontiguousarray(x, dtype=np.int8).reshape(-1) for x in heights27]
+        win = np.zeros(46 ** 3, np.uint8)
+        r = self.lib.hcref_light_rebuild(_ptr_array(b, ctypes.c_uint16), _ptr_array(h, ctypes.c_int8),
+                                         win.ctypes.data)
+        if r != 0:
+            raise RuntimeError(f"hcref_light_rebuild failed: {r}")
+        return win.reshape(46, 46, 46)
 
     def mesh(self, blocks27, light27):
         b, l = self._chunks(blocks27, light27)

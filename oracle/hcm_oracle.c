@@ -738,3 +738,215 @@ int orc_mesh_many(const orc_registry *reg, const uint16_t *blocks, const uint8_t
 		pthread_join(th[t], NULL);
 	return a.err;
 }
+
+/* ------------------------------------------------------------------------------------------------ */
+/* light rebuild before meshing (client/src/ChunkMeshTask.cpp:48-111)                                */
+/* ------------------------------------------------------------------------------------------------ */
+/* The mesh task's LightAlgo is BlockLightAlgo<Config{bound [-1, 17)^3}, Border 14>
+ * (client/include/client/ChunkMeshTask.inl:53-56): in_bound_bordered is [-15, 31)^3 and
+ * IsBorderLightInterfere compares a level with the L1 distance to the bound (BlockLightAlgo.hpp:25-42). */
+static inline uint32_t lw_index(int x, int y, int z) {
+	return (uint32_t)(x + 15) + ORC_LIGHT_W * ((uint32_t)(z + 15) + ORC_LIGHT_W * (uint32_t)(y + 15));
+}
+static inline int lw_in_bound(int x, int y, int z) {
+	return x >= -15 && x < 31 && y >= -15 && y < 31 && z >= -15 && z < 31;
+}
+static inline unsigned axis_dist(int v) { return v < -1 ? (unsigned)(-1 - v) : (v >= 17 ? (unsigned)(v - 17) : 0u); }
+static inline int light_interferes(int x, int y, int z, unsigned lvl) { /* BlockLightAlgo.hpp:32-42 */
+	return lvl > axis_dist(x) + axis_dist(y) + axis_dist(z);
+}
+
+/* FIFO of (position, level) entries (std::queue in BlockLightAlgo.hpp:22); a position is pushed at most
+ * once per level it is raised to, so 16 entries per cell bound the queue. */
+typedef struct lqueue {
+	uint32_t *e;
+	uint32_t head, tail, mask;
+} lqueue;
+static inline void lq_push(lqueue *q, int x, int y, int z, unsigned lvl) {
+	q->e[q->tail++ & q->mask] = (uint32_t)(x + 15) | (uint32_t)(y + 15) << 6 | (uint32_t)(z + 15) << 12 | lvl << 18;
+}
+
+typedef struct lctx {
+	const orc_registry *reg;
+	const uint8_t *light_level;
+	const uint16_t *const *blocks;
+	const int8_t *const *heights;
+	uint8_t *win;
+} lctx;
+
+static inline uint16_t lblock(const lctx *c, int x, int y, int z) { return c->blocks[slot_of(x, y, z)][local_of(x, y, z)]; }
+static inline unsigned wsun(const lctx *c, int x, int y, int z) { return c->win[lw_index(x, y, z)] & 15u; }
+
+/* BlockLightAlgo::PropagateLight (BlockLightAlgo.hpp:44-73) on one channel (shift 0: sunlight, 4: torchlight) */
+static void propagate(lctx *c, lqueue *q, int shift) {
+	while (q->head != q->tail) {
+		const uint32_t e = q->e[q->head++ & q->mask];
+		unsigned lvl = e >> 18;
+		if (lvl <= 1u)
+			continue;
+		--lvl;
+		const int x = (int)(e & 63u) - 15, y = (int)((e >> 6) & 63u) - 15, z = (int)((e >> 12) & 63u) - 15;
+		for (int f = 0; f < 6; ++f) { /* BlockFaceProceed (block/include/block/BlockFace.hpp:15-17) */
+			int n[3] = {x, y, z};
+			n[f >> 1] += 1 - ((f & 1) << 1);
+			if (!lw_in_bound(n[0], n[1], n[2]) || !light_interferes(n[0], n[1], n[2], lvl))
+				continue;
+			uint8_t *w = &c->win[lw_index(n[0], n[1], n[2])];
+			const unsigned cur = (*w >> shift) & 15u;
+			if (lvl > cur && indirect_pass(&c->reg->rows[c->reg->row_of[lblock(c, n[0], n[1], n[2])]])) {
+				*w = (uint8_t)((*w & ~(15u << shift)) | lvl << shift);
+				if (lvl > 1)
+					lq_push(q, n[0], n[1], n[2], lvl);
+			}
+		}
+	}
+}
+
+static int light_rebuild_with(lctx *c, lqueue *sq, lqueue *tq) {
+	sq->head = sq->tail = tq->head = tq->tail = 0;
+	/* initial torchlight and sunlight, torchlight seeds (ChunkMeshTask.cpp:49-59) */
+	for (int y = -15; y < 31; ++y)
+		for (int z = -15; z < 31; ++z)
+			for (int x = -15; x < 31; ++x) {
+				const unsigned torch = c->light_level[lblock(c, x, y, z)] & 15u;
+				const int yl = (y + 16) % 16;
+				const unsigned sun = yl >= c->heights[slot_of(x, y, z)][(x + 16) % 16 + 16 * ((z + 16) % 16)] ? 15u : 0u;
+				c->win[lw_index(x, y, z)] = (uint8_t)(sun | torch << 4);
+				if (light_interferes(x, y, z, torch))
+					lq_push(tq, x, y, z, torch);
+			}
+	/* sunlight seeds, skipping cells whose six neighbours are all full sunlight (:61-87) */
+	for (int y = -15; y < 31; ++y)
+		for (int z = -15; z < 31; ++z)
+			for (int x = -15; x < 31; ++x) {
+				const unsigned sun = wsun(c, x, y, z);
+				if (!light_interferes(x, y, z, sun))
+					continue;
+				if ((x == -15 || wsun(c, x - 1, y, z) == 15) && (x == 30 || wsun(c, x + 1, y, z) == 15) &&
+				    (z == -15 || wsun(c, x, y, z - 1) == 15) && (z == 30 || wsun(c, x, y, z + 1) == 15) &&
+				    (y == -15 || wsun(c, x, y - 1, z) == 15) && (y == 30 || wsun(c, x, y + 1, z) == 15))
+					continue;
+				lq_push(sq, x, y, z, sun);
+			}
+	propagate(c, sq, 0); /* :90-100 */
+	propagate(c, tq, 4); /* :101-111 */
+	return ORC_OK;
+}
+
+static void halo_of_window(const uint8_t *win, uint8_t *halo) {
+	for (int y = -1; y < 17; ++y)
+		for (int z = -1; z < 17; ++z)
+			for (int x = -1; x < 17; ++x)
+				halo[(x + 1) + ORC_HALO * ((z + 1) + ORC_HALO * (y + 1))] = win[lw_index(x, y, z)];
+}
+
+#define LQ_CAP (1u << 21) /* > 16 entries per cell of the 46^3 buffer */
+
+int orc_light_rebuild(const orc_registry *reg, const uint8_t *light_level, const uint16_t *const blocks[27],
+                      const int8_t *const heights[27], uint8_t *win, uint8_t *halo) {
+	lqueue sq = {(uint32_t *)malloc(4u * LQ_CAP), 0, 0, LQ_CAP - 1}, tq = {(uint32_t *)malloc(4u * LQ_CAP), 0, 0, LQ_CAP - 1};
+	if (!sq.e || !tq.e) {
+		free(sq.e), free(tq.e);
+		return ORC_ERR_CAPACITY;
+	}
+	lctx c = {reg, light_level, blocks, heights, win};
+	int r = light_rebuild_with(&c, &sq, &tq);
+	if (halo)
+		halo_of_window(win, halo);
+	free(sq.e), free(tq.e);
+	return r;
+}
+
+/* The 27 light arrays the mesher reads, from the rebuilt buffer (cells outside it stay 0; the mesher reads
+ * only [-1, 17)^3, ChunkMeshTask.cpp:120-121). */
+static void light_arrays_of_window(const uint8_t *win, uint8_t (*light)[ORC_CHUNK_VOXELS]) {
+	memset(light, 0, 27u * ORC_CHUNK_VOXELS);
+	for (int y = -15; y < 31; ++y)
+		for (int z = -15; z < 31; ++z)
+			for (int x = -15; x < 31; ++x)
+				light[slot_of(x, y, z)][local_of(x, y, z)] = win[lw_index(x, y, z)];
+}
+
+typedef struct relight_arg {
+	const orc_registry *reg;
+	const uint8_t *light_level;
+	const uint16_t *blocks;
+	const int8_t *heights;
+	const uint32_t *nbr;
+	uint32_t n_jobs;
+	int mesh;
+	uint8_t *halo_out;
+	uint64_t *quads_out;
+	uint32_t next;
+	int err;
+} relight_arg;
+
+static void *relight_worker(void *p) {
+	relight_arg *a = (relight_arg *)p;
+	lqueue sq = {(uint32_t *)malloc(4u * LQ_CAP), 0, 0, LQ_CAP - 1}, tq = {(uint32_t *)malloc(4u * LQ_CAP), 0, 0, LQ_CAP - 1};
+	uint8_t *win = (uint8_t *)malloc(ORC_LIGHT_WVOX);
+	uint8_t(*light)[ORC_CHUNK_VOXELS] = (uint8_t(*)[ORC_CHUNK_VOXELS])malloc(27u * ORC_CHUNK_VOXELS);
+	scratch *s = a->mesh ? (scratch *)malloc(sizeof(scratch)) : NULL;
+	orc_mesh_out out;
+	memset(&out, 0, sizeof(out));
+	if (a->mesh) {
+		out.vert_cap = 8u * MESH_VCAP;
+		out.idx_cap = 8u * MESH_ICAP;
+		out.verts = (uint32_t *)malloc(8 * (size_t)out.vert_cap);
+		out.idx = (uint16_t *)malloc(2 * (size_t)out.idx_cap);
+	}
+	if (!sq.e || !tq.e || !win || !light || (a->mesh && (!s || !out.verts || !out.idx))) {
+		__atomic_store_n(&a->err, ORC_ERR_CAPACITY, __ATOMIC_RELAXED);
+	} else {
+		for (;;) {
+			uint32_t j = __atomic_fetch_add(&a->next, 1u, __ATOMIC_RELAXED);
+			if (j >= a->n_jobs)
+				break;
+			const uint16_t *b[27];
+			const int8_t *h[27];
+			for (int k = 0; k < 27; ++k) {
+				b[k] = a->blocks + (size_t)a->nbr[27 * (size_t)j + k] * ORC_CHUNK_VOXELS;
+				h[k] = a->heights + (size_t)a->nbr[27 * (size_t)j + k] * (ORC_CHUNK * ORC_CHUNK);
+			}
+			lctx c = {a->reg, a->light_level, b, h, win};
+			light_rebuild_with(&c, &sq, &tq);
+			if (a->halo_out)
+				halo_of_window(win, a->halo_out + (size_t)j * ORC_HALO_VOX);
+			if (a->mesh) {
+				light_arrays_of_window(win, light);
+				const uint8_t *l[27];
+				for (int k = 0; k < 27; ++k)
+					l[k] = light[k];
+				int r = mesh_with(s, a->reg, b, l, &out);
+				if (r)
+					__atomic_store_n(&a->err, r, __ATOMIC_RELAXED);
+				uint64_t q = 0;
+				for (uint32_t m = 0; m < out.n_meshes; ++m)
+					q += out.n_verts[m] / 4;
+				if (a->quads_out)
+					a->quads_out[j] = q;
+			}
+		}
+	}
+	free(sq.e), free(tq.e), free(win), free(light), free(s), free(out.verts), free(out.idx);
+	return NULL;
+}
+
+int orc_relight_many(const orc_registry *reg, const uint8_t *light_level, const uint16_t *blocks,
+                     const int8_t *heights, const uint32_t *nbr, uint32_t n_jobs, uint32_t n_threads, int mesh,
+                     uint8_t *halo_out, uint64_t *quads_out) {
+	relight_arg a = {reg, light_level, blocks, heights, nbr, n_jobs, mesh, halo_out, quads_out, 0, 0};
+	if (n_threads < 1)
+		n_threads = 1;
+	if (n_threads > 256)
+		n_threads = 256;
+	pthread_t th[256];
+	uint32_t started = 0;
+	for (uint32_t t = 1; t < n_threads; ++t)
+		if (pthread_create(&th[started], NULL, relight_worker, &a) == 0)
+			++started;
+	relight_worker(&a);
+	for (uint32_t t = 0; t < started; ++t)
+		pthread_join(th[t], NULL);
+	return a.err;
+}

@@ -118,6 +118,29 @@ int orc_mesh_chunk(const orc_registry *reg, const uint16_t *const blocks[27], co
 int orc_mesh_many(const orc_registry *reg, const uint16_t *blocks, const uint8_t *light, const uint32_t *nbr,
                   uint32_t n_jobs, uint32_t n_threads, uint64_t *quads_out);
 
+/* ---- light rebuild before meshing (client/src/ChunkMeshTask.cpp:48-111) ---- */
+#define ORC_LIGHT_W 46 /* the extended light buffer spans [-15, 31) per axis (ChunkMeshTask.cpp:10-12) */
+#define ORC_LIGHT_WVOX (ORC_LIGHT_W * ORC_LIGHT_W * ORC_LIGHT_W)
+#define ORC_HALO 18 /* the mesher reads light over [-1, 17) per axis */
+#define ORC_HALO_VOX (ORC_HALO * ORC_HALO * ORC_HALO)
+
+/* Rebuilds the mesh task's extended light buffer: torchlight from Block::GetLightLevel, sunlight 15 where
+ * y >= the owning chunk's sunlight height, border seeds, then sunlight and torchlight propagation.
+ *   heights[27]  per chunk 16 x 16 InnerPos1 sunlight heights, index x + 16 z (Chunk.hpp:65-76, 96)
+ *   light_level  u8[65536], Block::GetLightLevel by Block::m_data (Block.hpp:71)
+ *   win          out, ORC_LIGHT_WVOX light bytes (sun | torch << 4), index (x+15) + 46 ((z+15) + 46 (y+15))
+ *   halo         out (may be NULL), the ORC_HALO_VOX bytes the mesher reads, index (x+1) + 18 ((z+1) + 18 (y+1))
+ * Returns ORC_OK or ORC_ERR_CAPACITY (allocation failure). */
+int orc_light_rebuild(const orc_registry *reg, const uint8_t *light_level, const uint16_t *const blocks[27],
+                      const int8_t *const heights[27], uint8_t *win, uint8_t *halo);
+
+/* Many jobs on a thread pool (the CPU baseline of the whole mesh task, ChunkMeshTask.cpp:41-127):
+ * light rebuild per job, and when mesh != 0 also the mesher over the rebuilt light. heights holds 256
+ * InnerPos1 per chunk index. halo_out [n_jobs][ORC_HALO_VOX] and quads_out [n_jobs] may be NULL. */
+int orc_relight_many(const orc_registry *reg, const uint8_t *light_level, const uint16_t *blocks,
+                     const int8_t *heights, const uint32_t *nbr, uint32_t n_jobs, uint32_t n_threads, int mesh,
+                     uint8_t *halo_out, uint64_t *quads_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -8,6 +8,11 @@
 //                            array instead of the rebuilt light buffer);
 //   hcref_mesh_many       -- the same over many jobs on a std::thread pool (CPU baseline, one chunk per task
 //                            like the reference's worker threads, client/src/WorldWorker.cpp:18-27);
+//   hcref_light_rebuild   -- the mesh task's light rebuild (client/src/ChunkMeshTask.cpp:48-111): the two
+//                            initialisation loops are restated here around the reference's own
+//                            BlockLightAlgo<Config, 14> (client/include/client/BlockLightAlgo.hpp,
+//                            ChunkMeshTask.inl:53-56), whose seed test and PropagateLight do the work;
+//   hcref_export_light_levels -- Block::GetLightLevel for all 65,536 Block::m_data values;
 //   hcref_export_registry -- writes the reference's 65,536-entry block property table
 //                            (block/generator/Block.cpp.in:30-32) and texture flags
 //                            (texture/internal/generated/block_texture_transparency.inl:1-3) in the
@@ -23,6 +28,7 @@
 #include <unordered_map>
 #include <vector>
 
+#include <client/BlockLightAlgo.hpp>
 #include <client/BlockMeshAlgo.hpp>
 
 #include "hcm_oracle.h"
@@ -255,6 +261,73 @@ extern "C" int hcref_export_registry(const char *path) {
 	uint16_t liq_tex = hc::texture::BlockTextures::kWater;
 	std::fwrite(liq, 1, 2, fp);
 	std::fwrite(&liq_tex, 2, 1, fp);
+	std::fclose(fp);
+	return 0;
+}
+
+namespace {
+using LightConfig = hc::client::BlockAlgoConfig<int8_t, hc::client::BlockAlgoBound<int8_t>{-1, -1, -1, 17, 17, 17}>;
+using LightAlgo = hc::client::BlockLightAlgo<LightConfig, 14>;
+inline uint32_t ext_index(int x, int y, int z) { return uint32_t(x + 15) + (uint32_t(z + 15) + uint32_t(y + 15) * 46u) * 46u; }
+} // namespace
+
+extern "C" int hcref_light_rebuild(const uint16_t *const blocks[27], const int8_t *const heights[27], uint8_t *win) {
+	auto block_at = [&](auto x, auto y, auto z) -> Block {
+		Block b;
+		b.SetData(blocks[slot_of(x, y, z)][local_of(x, y, z)]);
+		return b;
+	};
+	auto sun_at = [&](int x, int y, int z) -> bool { // Chunk::GetSunlightFromNeighbour (Chunk.hpp:73-79)
+		return ((y + 16) % 16) >= heights[slot_of(x, y, z)][(x + 16) % 16 + ((z + 16) % 16) * 16];
+	};
+	std::vector<Light> buf(46 * 46 * 46);
+	LightAlgo::Queue sun_q, torch_q;
+	for (int8_t y = -15; y < 31; ++y)
+		for (int8_t z = -15; z < 31; ++z)
+			for (int8_t x = -15; x < 31; ++x) {
+				Light light = {};
+				light.SetTorchlight(block_at(x, y, z).GetLightLevel());
+				light.SetSunlight(sun_at(x, y, z) ? 15 : 0);
+				buf[ext_index(x, y, z)] = light;
+				if (LightAlgo::IsBorderLightInterfere(x, y, z, light.GetTorchlight()))
+					torch_q.push({{x, y, z}, light.GetTorchlight()});
+			}
+	auto sun = [&](int x, int y, int z) { return buf[ext_index(x, y, z)].GetSunlight(); };
+	for (int8_t y = -15; y < 31; ++y)
+		for (int8_t z = -15; z < 31; ++z)
+			for (int8_t x = -15; x < 31; ++x) {
+				auto s = sun(x, y, z);
+				if (LightAlgo::IsBorderLightInterfere(x, y, z, s)) {
+					if ((x == -15 || sun(x - 1, y, z) == 15) && (x == 30 || sun(x + 1, y, z) == 15) &&
+					    (z == -15 || sun(x, y, z - 1) == 15) && (z == 30 || sun(x, y, z + 1) == 15) &&
+					    (y == -15 || sun(x, y - 1, z) == 15) && (y == 30 || sun(x, y + 1, z) == 15))
+						continue;
+					sun_q.push({{x, y, z}, s});
+				}
+			}
+	LightAlgo algo{};
+	algo.PropagateLight(
+	    &sun_q, block_at, [&](auto x, auto y, auto z) { return buf[ext_index(x, y, z)].GetSunlight(); },
+	    [&](auto x, auto y, auto z, hc::block::LightLvl l) { buf[ext_index(x, y, z)].SetSunlight(l); });
+	algo.PropagateLight(
+	    &torch_q, block_at, [&](auto x, auto y, auto z) { return buf[ext_index(x, y, z)].GetTorchlight(); },
+	    [&](auto x, auto y, auto z, hc::block::LightLvl l) { buf[ext_index(x, y, z)].SetTorchlight(l); });
+	for (size_t i = 0; i < buf.size(); ++i)
+		win[i] = buf[i].GetData();
+	return 0;
+}
+
+extern "C" int hcref_export_light_levels(const char *path) {
+	static uint8_t lv[65536];
+	for (uint32_t data = 0; data < 65536; ++data) {
+		Block b;
+		b.SetData((uint16_t)data);
+		lv[data] = (uint8_t)b.GetLightLevel();
+	}
+	FILE *fp = std::fopen(path, "wb");
+	if (!fp)
+		return ORC_ERR_IO;
+	std::fwrite(lv, 1, sizeof(lv), fp);
 	std::fclose(fp);
 	return 0;
 }

@@ -98,3 +98,39 @@ def slab(value=1, top=8):
 
 def uniform(value, light_byte=0x5A):
     return single([np.full(4096, value, np.uint16)] * 27, [np.full(4096, light_byte, np.uint8)] * 27)
+
+
+# ---- light rebuild inputs ----
+def random_heights(seed, n_chunks):
+    """Seeded sunlight heights in [0, 16] per chunk column (InnerPos1, index x + 16 z)."""
+    r = stream(seed, (n_chunks * 256 + 4095) // 4096, salt=0x4C)[:n_chunks * 256]
+    return ((r >> np.uint64(11)) % np.uint64(17)).astype(np.int8).reshape(n_chunks, 256)
+
+
+def lone_glowstone(pos=(8, 7, 9)):
+    """One glowstone (torchlight 15) in an all-air neighbourhood with no sunlight (heights 16)."""
+    c = np.zeros(4096, np.uint16)
+    x, y, z = pos
+    c[x + 16 * (z + 16 * y)] = 0x18
+    air = np.zeros(4096, np.uint16)
+    blocks = np.stack([c if s == 26 else air for s in range(27)])
+    return blocks, np.full((27, 256), 16, np.int8), np.arange(27, dtype=np.uint32).reshape(1, 27)
+
+
+def light27_from_window(win):
+    """The 27 light arrays over a rebuilt (46, 46, 46) [y, z, x] light window on [-15, 31); cells outside
+    the window stay 0 (the mesher reads only [-1, 17))."""
+    from hypercraft_amd.workloads import SLOT_OFFSETS
+    out = np.zeros((27, 16, 16, 16), np.uint8)  # [slot, y, z, x]
+    for s in range(27):
+        o = SLOT_OFFSETS[s] * 16  # chunk origin (x, y, z) relative to the centre chunk
+        lo = np.maximum(o, -15)
+        hi = np.minimum(o + 16, 31)
+        out[s, lo[1] - o[1]:hi[1] - o[1], lo[2] - o[2]:hi[2] - o[2], lo[0] - o[0]:hi[0] - o[0]] = \
+            win[lo[1] + 15:hi[1] + 15, lo[2] + 15:hi[2] + 15, lo[0] + 15:hi[0] + 15]
+    return [out[s].reshape(-1) for s in range(27)]
+
+
+def halo_of_window(win):
+    """The (18, 18, 18) [y, z, x] cells over [-1, 17) of a (46, 46, 46) window over [-15, 31)."""
+    return np.ascontiguousarray(win[14:32, 14:32, 14:32])

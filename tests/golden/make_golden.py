@@ -8,8 +8,12 @@ compiled from its own sources). The fixtures are data only:
   golden.npz    for each workload set: the SHA-256 of its inputs, and per job the reference's
                 quad count and the digest of its exact mesh list (tests/meshdigest.py);
                 for the single terrain chunk also the full reference meshes.
+  light.npz     the reference's light levels (Block::GetLightLevel by Block::m_data) and, for each
+                light set, per job the SHA-256 of the rebuilt light over the mesher's halo and the
+                mesh digest over that light (the mesh task, client/src/ChunkMeshTask.cpp:41-127).
 
     python tests/golden/make_golden.py [set ...]   (no names: every set)
+    python tests/golden/make_golden.py light
 """
 import os
 import sys
@@ -51,6 +55,45 @@ def golden_sets():
     return sets
 
 
+def light_sets():
+    """name -> (blocks, heights, nbr) for the light rebuild before meshing. Shared with the tests."""
+    sets = {
+        "lit_glowstone": cases.lone_glowstone(),
+        "lit_terrain_chunk": workloads.lit_terrain_chunk(),
+        "lit_cave_region": workloads.lit_cave_region(),
+    }
+    for i, fam in enumerate(("all_values", "glassy", "custom", "flow")):
+        blocks, _, nbr = cases.palette(fam, 31 + i, 8, density=0.3)
+        blocks = workloads.place_glowstone(blocks, 41 + i, 0.003)
+        sets["lit_palette_" + fam] = (blocks, cases.random_heights(51 + i, blocks.shape[0]), nbr)
+    return sets
+
+
+def main_light():
+    """Writes light.npz: the reference's light levels, and per light set and job the SHA-256 of the
+    rebuilt light over the mesher's 18^3 halo and the mesh digest over the rebuilt light."""
+    import hashlib
+
+    ref = cpu_oracle.Reference()
+    out = {"light_levels": ref.light_levels()}
+    for name, (blocks, heights, nbr) in light_sets().items():
+        n = nbr.shape[0]
+        hd = np.zeros((n, 32), np.uint8)
+        md = np.zeros((n, 32), np.uint8)
+        for j in range(n):
+            b27 = [blocks[s] for s in nbr[j]]
+            win = ref.light_rebuild(b27, workloads.slice_heights(heights, nbr, j))
+            hd[j] = np.frombuffer(hashlib.sha256(cases.halo_of_window(win).tobytes()).digest(), np.uint8)
+            md[j] = np.frombuffer(meshdigest.digest(ref.mesh(b27, cases.light27_from_window(win))), np.uint8)
+            if name in ("lit_glowstone", "lit_terrain_chunk"):
+                out[name + "_window"] = win
+        out[name + "_halo_digest"] = hd
+        out[name + "_mesh_digest"] = md
+        out[name + "_input"] = np.frombuffer(meshdigest.input_digest(blocks, heights, nbr).encode(), np.uint8)
+        print(f"{name}: {n} jobs")
+    np.savez_compressed(os.path.join(HERE, "light.npz"), **out)
+
+
 def main(only=None):
     """Writes every set, or with names given only those (merged into the existing golden.npz)."""
     ref = cpu_oracle.Reference()
@@ -84,4 +127,7 @@ def main(only=None):
 
 
 if __name__ == "__main__":
-    main(sys.argv[1:])
+    if sys.argv[1:] == ["light"]:
+        main_light()
+    else:
+        main(sys.argv[1:])
