@@ -1,8 +1,9 @@
 // hcm_api.cpp -- host side of the C ABI declared in include/hcm.h.
 //
 // Owns device memory (registry tables, quad pools, chunk stores, job tables), the HIP stream each
-// context works on, and the HIP graph that replays the batched pipeline (counter reset + persistent
-// meshing kernel). Kernels live in hcm_kernels.hip.
+// context works on, and the HIP graph that replays the batched pipeline (counter resets, the light
+// rebuild when asked for, the persistent meshing kernel). Kernels live in hcm_kernels.hip and
+// hcm_light.hip.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -41,6 +42,7 @@ struct DeviceTables {
 	std::vector<uint32_t> codes;
 	std::vector<hcm_custom_mesh> meshes;
 	std::vector<uint8_t> tex_flags;
+	std::vector<uint8_t> light_info; // [65536]: light level | kLightInfoPass
 	uint32_t flow_pid_base = 0, flow_tex = 0, src_id = 0, flow_id = 0;
 };
 
@@ -145,6 +147,14 @@ int build_tables(const hcm_registry *r, DeviceTables *t) {
 			}
 		t->pid_of[65536 + id] = v;
 	}
+	// the light rebuild's view of a block: Block::GetLightLevel and GetIndirectLightPass (Block.hpp:70-71)
+	t->light_info.resize(65536);
+	for (uint32_t d = 0; d < 65536; ++d) {
+		const uint32_t lv = r->light_level ? r->light_level[d] : 0u;
+		if (lv > 15)
+			return HCM_ERR_INVALID;
+		t->light_info[d] = uint8_t(lv | (r->rows[r->row_of[d]].transparency != 0 ? kLightInfoPass : 0u));
+	}
 	t->meshes.assign(r->meshes, r->meshes + r->n_meshes);
 	t->tex_flags.assign(r->tex_flags, r->tex_flags + r->n_tex);
 	t->flow_tex = r->liquid_flow_tex;
@@ -171,6 +181,7 @@ struct hcm_ctx {
 	uint32_t *d_codes = nullptr;
 	hcm_custom_mesh *d_meshes = nullptr;
 	uint8_t *d_tex_flags = nullptr;
+	uint8_t *d_light_info = nullptr;
 	// batched pool
 	uint64_t pool_quads = 0;
 	uint2 *d_verts = nullptr;
@@ -194,6 +205,9 @@ struct hcm_store {
 	uint32_t n_slots = 0;
 	uint16_t *d_blocks = nullptr;
 	uint8_t *d_light = nullptr;
+	int8_t *d_heights = nullptr;  // [n_slots][256] sunlight heights
+	uint16_t *d_packed = nullptr; // [n_slots][kLightPlanes][256] light-rebuild inputs, packed per batch run
+	uint32_t *d_levels = nullptr; // [n_slots] torchlight levels present
 };
 
 struct hcm_batch {
@@ -211,6 +225,13 @@ struct hcm_batch {
 	uint32_t graph_threads = 0, graph_grid = 0;
 	uint32_t last_mode = 0;
 	bool ran = false;
+	// light rebuild (HCM_LIGHT_REBUILD)
+	uint32_t light_mode = HCM_LIGHT_STORE, graph_light = HCM_LIGHT_STORE;
+	uint32_t n_read_slots = 0;
+	uint32_t *d_read_slots = nullptr; // the distinct store slots the jobs read
+	Counters *d_lctr = nullptr;
+	uint8_t *d_halo_light = nullptr;  // [n_jobs][kLightJobBytes]
+	bool light_ran = false;
 };
 
 extern "C" int hcm_last_hip_error(void) { return g_last_hip; }
@@ -242,6 +263,7 @@ static void free_ctx(hcm_ctx *c) {
 	dfree(c->d_codes);
 	dfree(c->d_meshes);
 	dfree(c->d_tex_flags);
+	dfree(c->d_light_info);
 	dfree(c->d_verts);
 	dfree(c->d_idx);
 	dfree(c->d_pblocks);
@@ -280,6 +302,8 @@ static int create_impl(hcm_ctx *c, const hcm_registry *registry, uint64_t pool_q
 	HCM_HIP(dmalloc(&c->d_codes, t.codes.size()));
 	HCM_HIP(dmalloc(&c->d_meshes, t.meshes.size()));
 	HCM_HIP(dmalloc(&c->d_tex_flags, t.tex_flags.size()));
+	HCM_HIP(dmalloc(&c->d_light_info, t.light_info.size()));
+	HCM_HIP(hipMemcpy(c->d_light_info, t.light_info.data(), t.light_info.size(), hipMemcpyHostToDevice));
 	HCM_HIP(hipMemcpy(c->d_pid_of, t.pid_of.data(), t.pid_of.size(), hipMemcpyHostToDevice));
 	HCM_HIP(hipMemcpy(c->d_rows, t.rows.data(), t.rows.size() * sizeof(uint2), hipMemcpyHostToDevice));
 	HCM_HIP(hipMemcpy(c->d_pid_vox, t.pid_vox.data(), t.pid_vox.size() * 4, hipMemcpyHostToDevice));
@@ -482,11 +506,11 @@ extern "C" int hcm_store_create(hcm_ctx *ctx, uint32_t n_slots, hcm_store **out)
 	s->ctx = ctx;
 	s->n_slots = n_slots;
 	if (dmalloc(&s->d_blocks, (size_t)n_slots * HCM_CHUNK_VOXELS) != hipSuccess ||
-	    dmalloc(&s->d_light, (size_t)n_slots * HCM_CHUNK_VOXELS) != hipSuccess) {
+	    dmalloc(&s->d_light, (size_t)n_slots * HCM_CHUNK_VOXELS) != hipSuccess ||
+	    dmalloc(&s->d_heights, (size_t)n_slots * 256) != hipSuccess ||
+	    hipMemset(s->d_heights, 0, (size_t)n_slots * 256) != hipSuccess) {
 		g_last_hip = (int)hipGetLastError();
-		dfree(s->d_blocks);
-		dfree(s->d_light);
-		delete s;
+		hcm_store_destroy(s);
 		return HCM_ERR_HIP;
 	}
 	*out = s;
@@ -499,6 +523,9 @@ extern "C" void hcm_store_destroy(hcm_store *s) {
 	(void)hipStreamSynchronize(s->ctx->stream);
 	dfree(s->d_blocks);
 	dfree(s->d_light);
+	dfree(s->d_heights);
+	dfree(s->d_packed);
+	dfree(s->d_levels);
 	delete s;
 }
 
@@ -510,6 +537,15 @@ extern "C" int hcm_store_upload(hcm_store *s, uint32_t first, uint32_t n, const 
 	                       hipMemcpyHostToDevice, st));
 	HCM_HIP(hipMemcpyAsync(s->d_light + (size_t)first * HCM_CHUNK_VOXELS, light, (size_t)n * HCM_CHUNK_VOXELS,
 	                       hipMemcpyHostToDevice, st));
+	HCM_HIP(hipStreamSynchronize(st));
+	return HCM_OK;
+}
+
+extern "C" int hcm_store_upload_sunlight(hcm_store *s, uint32_t first, uint32_t n, const int8_t *heights) {
+	if (!s || !heights || (uint64_t)first + n > s->n_slots)
+		return HCM_ERR_INVALID;
+	hipStream_t st = s->ctx->stream;
+	HCM_HIP(hipMemcpyAsync(s->d_heights + (size_t)first * 256, heights, (size_t)n * 256, hipMemcpyHostToDevice, st));
 	HCM_HIP(hipStreamSynchronize(st));
 	return HCM_OK;
 }
@@ -551,7 +587,48 @@ extern "C" int hcm_batch_create(hcm_ctx *ctx, hcm_store *store, const uint32_t *
 		return fail();
 	if (hipMemcpy(b->d_nbr, nbr, 27ull * n_jobs * 4, hipMemcpyHostToDevice) != hipSuccess)
 		return fail();
+	// the distinct store slots the jobs read, for the light rebuild's packing pass
+	std::vector<uint32_t> slots(nbr, nbr + 27ull * n_jobs);
+	std::sort(slots.begin(), slots.end());
+	slots.erase(std::unique(slots.begin(), slots.end()), slots.end());
+	b->n_read_slots = (uint32_t)slots.size();
+	if (dmalloc(&b->d_read_slots, slots.size()) != hipSuccess ||
+	    hipMemcpy(b->d_read_slots, slots.data(), slots.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
+		return fail();
 	*out = b;
+	return HCM_OK;
+}
+
+extern "C" int hcm_batch_set_light(hcm_batch *b, uint32_t light_mode) {
+	if (!b || (light_mode != HCM_LIGHT_STORE && light_mode != HCM_LIGHT_REBUILD))
+		return HCM_ERR_INVALID;
+	hcm_ctx *c = b->ctx;
+	std::lock_guard<std::mutex> lk(c->mu);
+	if (light_mode == HCM_LIGHT_REBUILD) {
+		hcm_store *st = b->store;
+		if (!st->d_packed) {
+			HCM_HIP(dmalloc(&st->d_packed, (size_t)st->n_slots * kLightPlanes * 256));
+			HCM_HIP(dmalloc(&st->d_levels, st->n_slots));
+		}
+		if (!b->d_lctr)
+			HCM_HIP(dmalloc(&b->d_lctr, 1));
+		if (!b->d_halo_light)
+			HCM_HIP(dmalloc(&b->d_halo_light, (size_t)b->n_jobs * kLightJobBytes));
+	}
+	b->light_mode = light_mode;
+	return HCM_OK;
+}
+
+extern "C" int hcm_batch_fetch_light(hcm_batch *b, uint32_t job, uint8_t *halo) {
+	if (!b || !halo || job >= b->n_jobs || !b->light_ran)
+		return HCM_ERR_INVALID;
+	std::vector<uint8_t> rows(kLightJobBytes);
+	hipStream_t st = b->ctx->stream;
+	HCM_HIP(hipMemcpyAsync(rows.data(), b->d_halo_light + (size_t)job * kLightJobBytes, kLightJobBytes,
+	                       hipMemcpyDeviceToHost, st));
+	HCM_HIP(hipStreamSynchronize(st));
+	for (uint32_t r = 0; r < 18 * 18; ++r) // row hz + 18 hy holds x = -1 .. 16
+		std::memcpy(halo + 18 * r, rows.data() + r * kLightRowBytes, 18);
 	return HCM_OK;
 }
 
@@ -572,6 +649,9 @@ extern "C" void hcm_batch_destroy(hcm_batch *b) {
 	dfree(b->d_ctr);
 	dfree(b->d_recs);
 	dfree(b->d_res);
+	dfree(b->d_read_slots);
+	dfree(b->d_lctr);
+	dfree(b->d_halo_light);
 	delete b;
 }
 
@@ -589,7 +669,38 @@ static MeshParams batch_params(hcm_batch *b) {
 	p.recs = b->d_recs;
 	p.rec_cap = b->rec_cap;
 	p.results = b->d_res;
+	p.halo_light = b->light_mode == HCM_LIGHT_REBUILD ? b->d_halo_light : nullptr;
 	return p;
+}
+
+static LightParams light_params(hcm_batch *b) {
+	LightParams lp;
+	std::memset(&lp, 0, sizeof(lp));
+	lp.blocks = b->store->d_blocks;
+	lp.heights = b->store->d_heights;
+	lp.light_info = b->ctx->d_light_info;
+	lp.slots = b->d_read_slots;
+	lp.n_slots = b->n_read_slots;
+	lp.n_jobs = b->n_jobs;
+	lp.packed = b->store->d_packed;
+	lp.levels = b->store->d_levels;
+	lp.nbr = b->d_nbr;
+	lp.ctr = b->d_lctr;
+	lp.halo_light = b->d_halo_light;
+	return lp;
+}
+
+// Light rebuild of every job of the batch: pack the read slots, then the persistent light kernel.
+static hipError_t enqueue_light(hcm_batch *b, hipStream_t st) {
+	const LightParams lp = light_params(b);
+	hipError_t e = launch_reset(b->d_lctr, st);
+	if (e == hipSuccess)
+		e = launch_light_prep(lp, st);
+	if (e == hipSuccess) {
+		const int per_cu = std::max(1, std::min(2, (160 * 1024) / std::max(1, light_lds_bytes())));
+		e = launch_light(lp, per_cu * b->ctx->n_cu, st);
+	}
+	return e;
 }
 
 extern "C" int hcm_batch_run(hcm_batch *b, uint32_t mode) {
@@ -604,15 +715,17 @@ extern "C" int hcm_batch_run(hcm_batch *b, uint32_t mode) {
 	MeshParams p = batch_params(b);
 	if (mode == HCM_RUN_PERSISTENT) {
 		const uint32_t grid = (uint32_t)persistent_grid(c, threads);
-		if (!b->graph_exec || b->graph_threads != threads || b->graph_grid != grid) {
+		if (!b->graph_exec || b->graph_threads != threads || b->graph_grid != grid || b->graph_light != b->light_mode) {
 			if (b->graph_exec)
 				(void)hipGraphExecDestroy(b->graph_exec), b->graph_exec = nullptr;
 			if (b->graph)
 				(void)hipGraphDestroy(b->graph), b->graph = nullptr;
 			p.persistent = 1;
 			HCM_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-			hipError_t e1 = launch_reset(b->d_ctr, st);
-			hipError_t e2 = launch_mesh(p, (int)threads, (int)grid, st);
+			hipError_t e1 = b->light_mode == HCM_LIGHT_REBUILD ? enqueue_light(b, st) : hipSuccess;
+			if (e1 == hipSuccess)
+				e1 = launch_reset(b->d_ctr, st);
+			hipError_t e2 = e1 == hipSuccess ? launch_mesh(p, (int)threads, (int)grid, st) : hipSuccess;
 			hipError_t e3 = hipStreamEndCapture(st, &b->graph);
 			if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
 				g_last_hip = (int)(e1 != hipSuccess ? e1 : (e2 != hipSuccess ? e2 : e3));
@@ -621,14 +734,17 @@ extern "C" int hcm_batch_run(hcm_batch *b, uint32_t mode) {
 			HCM_HIP(hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0));
 			b->graph_threads = threads;
 			b->graph_grid = grid;
+			b->graph_light = b->light_mode;
 		}
 		HCM_HIP(hipEventRecord(b->ev0, st));
 		HCM_HIP(hipGraphLaunch(b->graph_exec, st));
 		HCM_HIP(hipEventRecord(b->ev1, st));
 	} else {
+		HCM_HIP(hipEventRecord(b->ev0, st));
+		if (b->light_mode == HCM_LIGHT_REBUILD)
+			HCM_HIP(enqueue_light(b, st));
 		HCM_HIP(launch_reset(b->d_ctr, st));
 		p.persistent = 0;
-		HCM_HIP(hipEventRecord(b->ev0, st));
 		for (uint32_t j = 0; j < b->n_jobs; ++j) {
 			p.job_base = j;
 			HCM_HIP(launch_mesh(p, (int)threads, 1, st));
@@ -636,6 +752,7 @@ extern "C" int hcm_batch_run(hcm_batch *b, uint32_t mode) {
 		HCM_HIP(hipEventRecord(b->ev1, st));
 	}
 	b->ran = true;
+	b->light_ran = b->light_mode == HCM_LIGHT_REBUILD;
 	b->last_mode = mode;
 	return HCM_OK;
 }

@@ -72,6 +72,30 @@ struct MeshParams {
 	uint32_t rec_cap;
 	uint32_t pad;
 	hcm_job_result *results;
+	// [n_jobs][kLightJobBytes]: each job's rebuilt light over its halo, or null: light from the store
+	const uint8_t *halo_light;
+};
+
+// ---- light rebuild before meshing (client/src/ChunkMeshTask.cpp:48-111) ----
+// Per store slot, the inputs of the light rebuild packed as 16-bit x-rows (row z + 16 y, bit x):
+// plane 0 the indirect light pass, plane 1 sunlight, planes 2..5 the bits of the torchlight level.
+constexpr int kLightPlanes = 6;
+constexpr uint32_t kLightRowBytes = 20;                        // one halo x-row of rebuilt light (18 used)
+constexpr uint32_t kLightJobBytes = 18 * 18 * kLightRowBytes; // rows hz + 18 hy
+constexpr uint8_t kLightInfoPass = 0x10; // light_info: level | kLightInfoPass when the block passes light
+
+struct LightParams {
+	const uint16_t *blocks;    // store: slot * 4096 + local
+	const int8_t *heights;     // store: slot * 256 + x + 16 z
+	const uint8_t *light_info; // [65536] by Block::m_data
+	const uint32_t *slots;     // the store slots the batch reads
+	uint32_t n_slots;
+	uint32_t n_jobs;
+	uint16_t *packed;          // [store slots][kLightPlanes][256]
+	uint32_t *levels;          // [store slots]: bit l set when a voxel emits torchlight level l
+	const uint32_t *nbr;       // 27 slots per job
+	Counters *ctr;             // work queues of the persistent light kernel
+	uint8_t *halo_light;       // [n_jobs][kLightJobBytes]
 };
 
 // One voxel edit of a device chunk store: slot, chunk-local index, block, light.
@@ -84,6 +108,10 @@ hipError_t launch_reset(Counters *ctr, hipStream_t stream);
 hipError_t launch_mesh(const MeshParams &p, int threads, int grid, hipStream_t stream);
 hipError_t launch_edits(uint16_t *blocks, uint8_t *light, const VoxelEdit *edits, uint32_t n, hipStream_t stream);
 int mesh_lds_bytes(int threads);
+// Launchers (hcm_light.hip)
+hipError_t launch_light_prep(const LightParams &p, hipStream_t stream);
+hipError_t launch_light(const LightParams &p, int grid, hipStream_t stream);
+int light_lds_bytes();
 
 // Context internals the serving loop needs (hcm_api.cpp)
 struct CtxInfo {

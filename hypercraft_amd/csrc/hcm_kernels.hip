@@ -948,16 +948,26 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 		const size_t cr = (size_t)s.slots[slot27(16, y, z)] * HCM_CHUNK_VOXELS + lrow;
 		const uint4 b0 = *reinterpret_cast<const uint4 *>(p.blocks + cm);
 		const uint4 b1 = *reinterpret_cast<const uint4 *>(p.blocks + cm + 8);
-		const uint4 lm = *reinterpret_cast<const uint4 *>(p.light + cm);
-		const uint32_t bl = p.blocks[cl], br = p.blocks[cr], ll = p.light[cl], lr = p.light[cr];
 		const uint32_t bw[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
-		const uint32_t lw[4] = {lm.x, lm.y, lm.z, lm.w};
 		uint32_t blk[18], lt[18];
-		blk[0] = bl, lt[0] = ll, blk[17] = br, lt[17] = lr;
+		blk[0] = p.blocks[cl], blk[17] = p.blocks[cr];
 #pragma unroll
-		for (int x = 0; x < 16; ++x) {
+		for (int x = 0; x < 16; ++x)
 			blk[x + 1] = (bw[x >> 1] >> (16 * (x & 1))) & 0xffffu;
-			lt[x + 1] = (lw[x >> 2] >> (8 * (x & 3))) & 0xffu;
+		if (p.halo_light) { // the job's rebuilt light, one 20-byte record per halo x-row
+			const uint32_t *hl = reinterpret_cast<const uint32_t *>(p.halo_light + (size_t)job * kLightJobBytes +
+			                                                        row * kLightRowBytes);
+			const uint32_t lw[5] = {hl[0], hl[1], hl[2], hl[3], hl[4]};
+#pragma unroll
+			for (int x = 0; x < 18; ++x)
+				lt[x] = (lw[x >> 2] >> (8 * (x & 3))) & 0xffu;
+		} else {
+			const uint4 lm = *reinterpret_cast<const uint4 *>(p.light + cm);
+			const uint32_t lw[4] = {lm.x, lm.y, lm.z, lm.w};
+			lt[0] = p.light[cl], lt[17] = p.light[cr];
+#pragma unroll
+			for (int x = 0; x < 16; ++x)
+				lt[x + 1] = (lw[x >> 2] >> (8 * (x & 3))) & 0xffu;
 		}
 		// one reference value per wavefront (its first row's first voxel): a halo of one block value has
 		// every voxel equal to it and all wavefronts' references equal

@@ -167,6 +167,7 @@ struct Prop {
 	uint8_t collision = kSolid;
 	bool has_mesh = false;
 	hcm_custom_mesh mesh{};
+	uint8_t light_level = 0; // BlockProperty::light_level (block/include/block/BlockProperty.hpp:22)
 };
 
 Prop cube(const Tex6 &t, uint8_t transparency = kOpaque, uint8_t collision = kSolid) {
@@ -249,7 +250,11 @@ std::array<IdDesc, 256> make_ids() {
 	single(23, [](uint8_t, uint8_t) {
 		return meshed(same(kNone), kTransp, kCollNone, cross_mesh(kBrownMushroom, 6, 0, 9, 1));
 	});
-	single(24, [](uint8_t, uint8_t) { return cube(same(kGlowstone)); });
+	single(24, [](uint8_t, uint8_t) { // glowstone.hpp: opaque, light level 15
+		Prop p = cube(same(kGlowstone));
+		p.light_level = 15;
+		return p;
+	});
 	return d;
 }
 
@@ -258,6 +263,7 @@ struct Built {
 	std::vector<hcm_block_row> rows;
 	std::vector<hcm_custom_mesh> meshes;
 	std::vector<uint8_t> tex_flags;
+	std::vector<uint8_t> light_level = std::vector<uint8_t>(65536);
 	hcm_registry reg{};
 
 	Built() {
@@ -275,6 +281,7 @@ struct Built {
 			uint8_t v = std::min<uint8_t>(uint8_t(meta & ((1u << vb) - 1u)), uint8_t(id.variants - 1u));
 			uint8_t t = std::min<uint8_t>(uint8_t((meta >> vb) & ((1u << tb) - 1u)), uint8_t(id.transforms - 1u));
 			Prop p = id.get ? id.get(v, t) : air_like();
+			light_level[data] = p.light_level;
 			hcm_block_row r;
 			std::memset(&r, 0, sizeof(r));
 			for (int f = 0; f < 6; ++f)
@@ -310,6 +317,7 @@ struct Built {
 		reg.liquid_source_id = 14; // WaterTrait::kSourceID = kWater (Water.hpp:6)
 		reg.liquid_flow_id = 15;   // WaterTrait::kFlowID = kFlowingWater
 		reg.liquid_flow_tex = kWater;
+		reg.light_level = light_level.data();
 	}
 };
 

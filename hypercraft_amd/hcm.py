@@ -24,6 +24,8 @@ HCM_ERR_BUSY = -6
 
 RUN_PERSISTENT = 0
 RUN_PER_CHUNK = 1
+LIGHT_STORE = 0    # mesh with the store's light bytes
+LIGHT_REBUILD = 1  # rebuild light from blocks and sunlight heights first (ChunkMeshTask.cpp:48-111)
 
 HCM_GEN_PLANTS = 1
 
@@ -53,7 +55,7 @@ class RegistryC(ctypes.Structure):
                 ("row_of", ctypes.POINTER(ctypes.c_uint16)), ("rows", ctypes.POINTER(BlockRowC)),
                 ("meshes", ctypes.POINTER(CustomMeshC)), ("tex_flags", ctypes.POINTER(ctypes.c_uint8)),
                 ("liquid_source_id", ctypes.c_uint8), ("liquid_flow_id", ctypes.c_uint8),
-                ("liquid_flow_tex", ctypes.c_uint16)]
+                ("liquid_flow_tex", ctypes.c_uint16), ("light_level", ctypes.POINTER(ctypes.c_uint8))]
 
 
 class MeshDescC(ctypes.Structure):
@@ -114,6 +116,7 @@ SIGNATURES = [
     ("hcm_store_destroy", None, [_VP]),
     ("hcm_store_upload", ctypes.c_int, [_VP, ctypes.c_uint32, ctypes.c_uint32, _VP, _VP]),
     ("hcm_store_device_ptrs", ctypes.c_int, [_VP, ctypes.POINTER(_VP), ctypes.POINTER(_VP)]),
+    ("hcm_store_upload_sunlight", ctypes.c_int, [_VP, ctypes.c_uint32, ctypes.c_uint32, _VP]),
     ("hcm_batch_create", ctypes.c_int, [_VP, _VP, _VP, ctypes.c_uint32, ctypes.POINTER(_VP)]),
     ("hcm_batch_destroy", None, [_VP]),
     ("hcm_batch_run", ctypes.c_int, [_VP, ctypes.c_uint32]),
@@ -122,6 +125,8 @@ SIGNATURES = [
     ("hcm_batch_fetch", ctypes.c_int, [_VP, ctypes.c_uint32, ctypes.POINTER(HostResultC)]),
     ("hcm_batch_totals", ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]),
     ("hcm_batch_download", ctypes.c_int, [_VP, _VP, _VP, ctypes.c_uint32, _VP, _VP, ctypes.c_uint64]),
+    ("hcm_batch_set_light", ctypes.c_int, [_VP, ctypes.c_uint32]),
+    ("hcm_batch_fetch_light", ctypes.c_int, [_VP, ctypes.c_uint32, _VP]),
     ("hcm_set_tuning", ctypes.c_int, [_VP, ctypes.POINTER(TuningC)]),
     ("hcm_get_tuning", ctypes.c_int, [_VP, ctypes.POINTER(TuningC)]),
     ("hcm_server_create", ctypes.c_int, [_VP, ctypes.POINTER(ServerConfigC), ctypes.POINTER(_VP)]),
@@ -286,6 +291,12 @@ class Store:
         check("hcm_store_upload", lib().hcm_store_upload(self._ptr, first, blocks.shape[0], blocks.ctypes.data,
                                                           light.ctypes.data))
 
+    def upload_sunlight(self, heights, first=0):
+        """Sunlight heights (n, 256) int8 per chunk (Chunk::m_sunlight_heights, index x + 16 z)."""
+        heights = np.ascontiguousarray(heights, np.int8).reshape(-1, 256)
+        check("hcm_store_upload_sunlight", lib().hcm_store_upload_sunlight(self._ptr, first, heights.shape[0],
+                                                                            heights.ctypes.data))
+
     def close(self):
         if self._ptr:
             lib().hcm_store_destroy(self._ptr)
@@ -312,6 +323,16 @@ class Batch:
 
     def run(self, mode=RUN_PERSISTENT):
         check("hcm_batch_run", lib().hcm_batch_run(self._ptr, mode))
+
+    def set_light(self, mode):
+        """LIGHT_STORE: mesh with the store's light bytes; LIGHT_REBUILD: rebuild light first."""
+        check("hcm_batch_set_light", lib().hcm_batch_set_light(self._ptr, mode))
+
+    def fetch_light(self, job):
+        """One job's rebuilt light over [-1, 17)^3 after a LIGHT_REBUILD run: (18, 18, 18) u8 [y, z, x]."""
+        out = np.zeros(18 ** 3, np.uint8)
+        check("hcm_batch_fetch_light", lib().hcm_batch_fetch_light(self._ptr, job, out.ctypes.data))
+        return out.reshape(18, 18, 18)
 
     def sync(self):
         ms = ctypes.c_float()
@@ -476,5 +497,6 @@ def registry_arrays():
     rows = np.frombuffer(ctypes.string_at(r.rows, 16 * r.n_rows), np.uint8).reshape(r.n_rows, 16).copy()
     meshes = np.frombuffer(ctypes.string_at(r.meshes, 400 * r.n_meshes), np.uint8).reshape(r.n_meshes, 400).copy()
     tex_flags = np.frombuffer(ctypes.string_at(r.tex_flags, r.n_tex), np.uint8).copy()
+    light_level = np.ctypeslib.as_array(r.light_level, shape=(65536,)).copy()
     return {"row_of": row_of, "rows": rows, "meshes": meshes, "tex_flags": tex_flags,
-            "liquid": (r.liquid_source_id, r.liquid_flow_id, r.liquid_flow_tex)}
+            "liquid": (r.liquid_source_id, r.liquid_flow_id, r.liquid_flow_tex), "light_level": light_level}

@@ -88,6 +88,9 @@ typedef struct hcm_registry {
 	const uint8_t *tex_flags; /* [n_tex], by texture id */
 	uint8_t liquid_source_id, liquid_flow_id; /* WaterTrait::kSourceID / kFlowID (Water.hpp:6) */
 	uint16_t liquid_flow_tex;                 /* WaterTrait::kFlowTex */
+	/* Block::GetLightLevel by Block::m_data (block/include/block/Block.hpp:71), [65536]; NULL: no block
+	 * emits light. Read only by the light rebuild before meshing (HCM_LIGHT_REBUILD). */
+	const uint8_t *light_level;
 } hcm_registry;
 
 /* The library's built-in copy of the reference registry (the block headers under block/block/,
@@ -148,6 +151,11 @@ void hcm_store_destroy(hcm_store *store);
 int hcm_store_upload(hcm_store *store, uint32_t first, uint32_t n, const uint16_t *blocks, const uint8_t *light);
 /* Device pointers of the slot arrays (blocks: n_slots*4096 u16, light: n_slots*4096 u8). */
 int hcm_store_device_ptrs(hcm_store *store, void **blocks, void **light);
+/* Copies the sunlight heights of n chunks (int8[n][256], host memory) into slots [first, first+n):
+ * Chunk::m_sunlight_heights (client/include/client/Chunk.hpp:65-76, 96), index x + 16 z; a voxel is
+ * sunlit when its chunk-local y >= the height. A new store holds 0 everywhere, Chunk's default (every
+ * voxel sunlit). Read only by the light rebuild (HCM_LIGHT_REBUILD). */
+int hcm_store_upload_sunlight(hcm_store *store, uint32_t first, uint32_t n, const int8_t *heights);
 
 typedef struct hcm_mesh_rec { /* device-side mesh record; 40 bytes (cf. ChunkMeshInfo, ChunkMesh.hpp:12-17) */
 	uint32_t aabb[6];
@@ -175,7 +183,8 @@ int hcm_batch_create(hcm_ctx *ctx, hcm_store *store, const uint32_t *nbr, uint32
 void hcm_batch_destroy(hcm_batch *batch);
 /* Enqueues one meshing pass over every job (resets the pool first). Asynchronous. */
 int hcm_batch_run(hcm_batch *batch, uint32_t mode);
-/* Waits for the last run; *kernel_ms receives the HIP-event time of its meshing kernels. */
+/* Waits for the last run; *kernel_ms receives the HIP-event time of its kernels (with HCM_LIGHT_REBUILD
+ * the light rebuild and the meshing). */
 int hcm_batch_sync(hcm_batch *batch, float *kernel_ms);
 /* Per-job results of the last run (host array of n_jobs). */
 int hcm_batch_results(hcm_batch *batch, hcm_job_result *results);
@@ -187,6 +196,17 @@ int hcm_batch_totals(hcm_batch *batch, uint64_t *quads, uint64_t *meshes);
  * the used part of the quad pool (up to quad_cap quads: 4 vertices and 6 indices each). */
 int hcm_batch_download(hcm_batch *batch, hcm_job_result *results, hcm_mesh_rec *recs, uint32_t rec_cap,
                        uint32_t (*vertices)[2], uint16_t *indices, uint64_t quad_cap);
+
+/* Where a batch's light comes from. The reference's mesh task rebuilds light before every mesh
+ * (client/src/ChunkMeshTask.cpp:48-111): torchlight from the blocks' light levels, sunlight from the
+ * chunks' sunlight heights, then propagation through blocks that pass light indirectly, over the
+ * chunk's [-15, 31)^3 neighbourhood. */
+#define HCM_LIGHT_STORE 0u   /* mesh with the light bytes held in the store (the default) */
+#define HCM_LIGHT_REBUILD 1u /* rebuild each job's light from blocks and sunlight heights, then mesh */
+int hcm_batch_set_light(hcm_batch *batch, uint32_t light_mode);
+/* After a run with HCM_LIGHT_REBUILD: one job's rebuilt light over the 18^3 cells [-1, 17)^3 the mesher
+ * reads, index (x+1) + 18 ((z+1) + 18 (y+1)), light bytes sun | torch << 4 (block/include/block/Light.hpp). */
+int hcm_batch_fetch_light(hcm_batch *batch, uint32_t job, uint8_t *halo);
 
 /* Launch shape of the meshing kernel. 0 = default. */
 typedef struct hcm_tuning {

@@ -1,11 +1,12 @@
 """The block registry: the library's built-in table against the one exported from the reference,
 plus the registry invariants of the reference's own tests (block/test/test_block.cpp:10-67)."""
+import os
 import struct
 
 import numpy as np
 import pytest
 
-from conftest import REGISTRY
+from conftest import GOLDEN, REGISTRY
 
 
 def load_fixture():
@@ -44,6 +45,12 @@ def test_builtin_registry_equals_reference_export(fixture, builtin):
     assert np.array_equal(builtin["meshes"], fixture["meshes"])
     assert np.array_equal(builtin["tex_flags"], fixture["tex_flags"])
     assert tuple(builtin["liquid"]) == tuple(fixture["liquid"])
+
+
+def test_builtin_light_levels_equal_reference_export(builtin):
+    # Block::GetLightLevel for every Block value, as the reference resolves it (tests/golden/light.npz)
+    want = np.load(os.path.join(GOLDEN, "light.npz"))["light_levels"]
+    assert np.array_equal(builtin["light_level"], want)
 
 
 def _row(reg, data):
