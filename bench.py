@@ -8,14 +8,17 @@ one batched meshing pass over all 4096 chunks (HIP-graph replay of counter reset
 kernel) with the autotuned workgroup shape. Output is checked bit-exact against the digests the
 reference mesher produced for the same chunks.
 
-Two more legs go into the same JSON line:
+Three more legs go into the same JSON line:
   mixed_batches  200 batches of log-uniform size in [1, 4096] drawn from the cave region and a
                  plant-covered terrain region: captured-graph runs vs per-chunk launches, bit-exact;
   serving        a spiral load trace (radius 24 columns x 16 chunks, ~29k terrain chunks, Poisson
                  arrivals at 20k chunks/s) with concurrent single-voxel edits at 2k/s through the
-                 continuous-batching server (16 GB vertex pool): p50/p99 request latency.
+                 continuous-batching server (16 GB vertex pool): p50/p99 request latency;
+  relight        the whole mesh task on a lit cave region: light rebuilt from the blocks' light levels
+                 and the chunks' sunlight heights, then meshing, bit-exact with the reference.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-plain] [--no-mixed] [--no-serve]
+                    [--no-relight]
 """
 import argparse
 import json
@@ -61,6 +64,66 @@ def cpu_baseline(blocks, light, nbr, threads, min_seconds=10.0):
             "sample": f"{passes} passes over the {nbr.shape[0]}-chunk batch ({chunks} chunks, {quads} quads), "
                       f"{dt:.2f} s wall, {threads} threads",
             "quads_per_s": round(quads / dt, 1)}
+
+
+def relight_leg(ctx, hcm, workloads, meshdigest, steps, cpu_threads, with_cpu):
+    """The whole mesh task on the lit cave region (the cave region with glowstone in 0.05% of its air and
+    sunlight heights from its columns): light rebuilt from blocks and heights (ChunkMeshTask.cpp:48-111),
+    then meshing, one captured graph per step. Bit-exact against the reference's mesh digests."""
+    import hashlib
+
+    blocks, heights, nbr = workloads.lit_cave_region()
+    golden = np.load(os.path.join(ROOT, "tests", "golden", "light.npz"))
+    n = nbr.shape[0]
+    store = hcm.Store(ctx, blocks.shape[0])
+    store.upload(blocks, np.zeros(blocks.shape, np.uint8))
+    store.upload_sunlight(heights)
+    batch = hcm.Batch(ctx, store, nbr)
+    batch.set_light(hcm.LIGHT_REBUILD)
+    for _ in range(3):
+        batch.run()
+    batch.sync()
+    ms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.run()
+        ms.append(batch.sync())
+    dt = time.perf_counter() - t0
+    quads, meshes = batch.totals()
+    out = batch.download()
+    bad = sum(meshdigest.digest(hcm.job_meshes(j, *out)) != golden["lit_cave_region_mesh_digest"][j].tobytes()
+              for j in range(n))
+    bad_light = sum(hashlib.sha256(batch.fetch_light(j).tobytes()).digest() !=
+                    golden["lit_cave_region_halo_digest"][j].tobytes() for j in range(0, n, 16))
+    batch.close()
+    store.close()
+    res = {"workload": "lit cave region: 4096 chunks, glowstone in 0.05% of the air, sunlight heights from the "
+                       "columns; light rebuild + meshing per step",
+           "chunks_per_s": round(n * steps / dt, 1), "ms_per_step": round(dt * 1e3 / steps, 4),
+           "kernel_ms": round(float(np.mean(ms)), 4), "quads_per_step": int(quads),
+           "bit_exact": bool(bad == 0 and bad_light == 0 and (out[0]["status"] == 0).all())}
+    if with_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import cpu_oracle
+
+        if cpu_oracle.Reference.available():
+            impl, kind = cpu_oracle.Reference(), "reference"
+        else:
+            impl, kind = cpu_oracle.Oracle(os.path.join(ROOT, "tests", "golden", "registry.bin")), "port"
+        sample = nbr[:1024]
+        passes = 0
+        t0 = time.perf_counter()
+        while True:
+            impl.relight_many(golden["light_levels"], blocks, heights, sample, cpu_threads)
+            passes += 1
+            cdt = time.perf_counter() - t0
+            if cdt >= 5.0:
+                break
+        res["cpu_baseline"] = {"value": round(passes * sample.shape[0] / cdt, 1), "unit": "chunks/s",
+                               "cores": cpu_threads, "kind": kind,
+                               "sample": f"{passes} passes over the first {sample.shape[0]} jobs, {cdt:.2f} s wall, "
+                                         f"light rebuild + meshing per job"}
+    return res
 
 
 def measured_traffic():
@@ -181,6 +244,7 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--no-relight", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (for counter calibration)")
     args = ap.parse_args()
@@ -282,6 +346,10 @@ def main():
     if not args.no_serve:
         log("serving trace")
         res["serving"] = serve_leg(ctx, hcm, workloads)
+    if not args.no_relight:
+        log("light rebuild + meshing")
+        res["relight"] = relight_leg(ctx, hcm, workloads, meshdigest, 20, min(16, os.cpu_count() or 1),
+                                     not args.no_cpu)
     if not args.no_cpu:
         log("CPU baseline")
         res["cpu_baseline"] = cpu_baseline(blocks, light, nbr, min(16, os.cpu_count() or 1))

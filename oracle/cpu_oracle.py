@@ -182,6 +182,8 @@ class Reference(_Base):
         self.lib.hcref_export_registry.argtypes = [ctypes.c_char_p]
         self.lib.hcref_export_light_levels.argtypes = [ctypes.c_char_p]
         self.lib.hcref_light_rebuild.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        self.lib.hcref_relight_many.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
+                                                ctypes.c_uint32, ctypes.c_void_p]
         self.out = _Out()
 
     @staticmethod
@@ -220,6 +222,18 @@ class Reference(_Base):
         if r != 0:
             raise RuntimeError(f"hcref_mesh_chunk failed: {r}")
         return self.out.meshes()
+
+    def relight_many(self, light_level, blocks, heights, nbr, n_threads, mesh=True, halo=False):
+        """The reference's whole mesh task per job (light rebuild, then meshing): -> (None, quads (J,)).
+        Same call shape as Oracle.relight_many (the reference resolves light levels itself)."""
+        assert mesh and not halo
+        blocks = np.ascontiguousarray(blocks, np.uint16)
+        heights = np.ascontiguousarray(heights, np.int8)
+        nbr = np.ascontiguousarray(nbr, np.uint32)
+        q = np.zeros(nbr.shape[0], np.uint64)
+        self.lib.hcref_relight_many(blocks.ctypes.data, heights.ctypes.data, nbr.ctypes.data, nbr.shape[0],
+                                    n_threads, q.ctypes.data)
+        return None, q
 
     def mesh_many(self, blocks, light, nbr, n_threads):
         blocks = np.ascontiguousarray(blocks, np.uint16)

@@ -12,6 +12,8 @@
 //                            initialisation loops are restated here around the reference's own
 //                            BlockLightAlgo<Config, 14> (client/include/client/BlockLightAlgo.hpp,
 //                            ChunkMeshTask.inl:53-56), whose seed test and PropagateLight do the work;
+//   hcref_relight_many    -- the whole mesh task (light rebuild, then Generate over the rebuilt light)
+//                            over many jobs on a std::thread pool (CPU baseline of the relit batch);
 //   hcref_export_light_levels -- Block::GetLightLevel for all 65,536 Block::m_data values;
 //   hcref_export_registry -- writes the reference's 65,536-entry block property table
 //                            (block/generator/Block.cpp.in:30-32) and texture flags
@@ -329,5 +331,55 @@ extern "C" int hcref_export_light_levels(const char *path) {
 		return ORC_ERR_IO;
 	std::fwrite(lv, 1, sizeof(lv), fp);
 	std::fclose(fp);
+	return 0;
+}
+
+// The whole mesh task per job (ChunkMeshTask.cpp:41-127): light rebuild, then Generate reading light
+// from the rebuilt buffer as the task's lambda does (:120-121). CPU baseline of the relit batch.
+extern "C" int hcref_relight_many(const uint16_t *blocks, const int8_t *heights, const uint32_t *nbr,
+                                  uint32_t n_jobs, uint32_t n_threads, uint64_t *quads_out) {
+	std::atomic<uint32_t> next{0};
+	auto worker = [&]() {
+		Algo *algo = new Algo{};
+		std::vector<uint8_t> win(46 * 46 * 46);
+		for (;;) {
+			uint32_t j = next.fetch_add(1);
+			if (j >= n_jobs)
+				break;
+			const uint16_t *b[27];
+			const int8_t *h[27];
+			for (int s = 0; s < 27; ++s) {
+				b[s] = blocks + (size_t)nbr[27 * (size_t)j + s] * ORC_CHUNK_VOXELS;
+				h[s] = heights + (size_t)nbr[27 * (size_t)j + s] * 256;
+			}
+			hcref_light_rebuild(b, h, win.data());
+			auto meshes = algo->Generate(
+			    [&](auto x, auto y, auto z) -> Block {
+				    Block bk;
+				    bk.SetData(b[slot_of(x, y, z)][local_of(x, y, z)]);
+				    return bk;
+			    },
+			    [&](auto x, auto y, auto z) -> Light {
+				    Light lt;
+				    lt.SetData(win[ext_index(x, y, z)]);
+				    return lt;
+			    });
+			uint64_t q = 0;
+			for (auto &m : meshes)
+				q += m.vertices.size() / 4;
+			if (quads_out)
+				quads_out[j] = q;
+		}
+		delete algo;
+	};
+	if (n_threads <= 1) {
+		worker();
+		return 0;
+	}
+	std::vector<std::thread> th;
+	for (uint32_t t = 0; t < n_threads; ++t)
+		th.emplace_back(worker);
+	for (auto &t : th)
+		t.join();
 	return 0;
 }
