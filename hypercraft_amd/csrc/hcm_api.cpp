@@ -197,6 +197,13 @@ struct hcm_ctx {
 	hcm_job_result *d_pres = nullptr;
 	uint16_t *h_pblocks = nullptr; // pinned staging
 	uint8_t *h_plight = nullptr;
+	// plain path light rebuild
+	int8_t *d_pheights = nullptr;
+	int8_t *h_pheights = nullptr;
+	uint16_t *d_ppacked = nullptr;
+	uint32_t *d_plevels = nullptr;
+	Counters *d_plctr = nullptr;
+	uint8_t *d_phalo = nullptr;
 	hcm_tuning tuning{0, 256, {0, 0}};
 };
 
@@ -278,6 +285,13 @@ static void free_ctx(hcm_ctx *c) {
 		(void)hipHostFree(c->h_pblocks);
 	if (c->h_plight)
 		(void)hipHostFree(c->h_plight);
+	dfree(c->d_pheights);
+	dfree(c->d_ppacked);
+	dfree(c->d_plevels);
+	dfree(c->d_plctr);
+	dfree(c->d_phalo);
+	if (c->h_pheights)
+		(void)hipHostFree(c->h_pheights);
 	if (c->stream)
 		(void)hipStreamDestroy(c->stream);
 	delete c;
@@ -342,6 +356,12 @@ static int create_impl(hcm_ctx *c, const hcm_registry *registry, uint64_t pool_q
 	HCM_HIP(dmalloc(&c->d_pres, 1));
 	HCM_HIP(hipHostMalloc(reinterpret_cast<void **>(&c->h_pblocks), 27 * HCM_CHUNK_VOXELS * 2));
 	HCM_HIP(hipHostMalloc(reinterpret_cast<void **>(&c->h_plight), 27 * HCM_CHUNK_VOXELS));
+	HCM_HIP(dmalloc(&c->d_pheights, 27 * 256));
+	HCM_HIP(hipHostMalloc(reinterpret_cast<void **>(&c->h_pheights), 27 * 256));
+	HCM_HIP(dmalloc(&c->d_ppacked, 27 * kLightPlanes * 256));
+	HCM_HIP(dmalloc(&c->d_plevels, 27));
+	HCM_HIP(dmalloc(&c->d_plctr, 1));
+	HCM_HIP(dmalloc(&c->d_phalo, kLightJobBytes));
 	uint32_t ident[27];
 	for (uint32_t i = 0; i < 27; ++i)
 		ident[i] = i;
@@ -492,6 +512,70 @@ extern "C" int hcm_mesh_chunk(hcm_ctx *ctx, const uint16_t *const blocks[27], co
 	hcm_job_result res;
 	HCM_HIP(hipMemcpyAsync(&res, ctx->d_pres, sizeof(res), hipMemcpyDeviceToHost, st));
 	HCM_HIP(hipStreamSynchronize(st));
+	return fetch_job(st, res, ctx->d_precs, ctx->d_pverts, ctx->d_pidx, out);
+}
+
+extern "C" int hcm_mesh_chunk_relight(hcm_ctx *ctx, const uint16_t *const blocks[27], const int8_t *const heights[27],
+                                      hcm_host_result *out, uint8_t *light_out) {
+	if (!ctx || !blocks || !heights || !out)
+		return HCM_ERR_INVALID;
+	for (int i = 0; i < 27; ++i)
+		if (!blocks[i] || !heights[i])
+			return HCM_ERR_INVALID;
+	std::lock_guard<std::mutex> lk(ctx->mu);
+	for (int i = 0; i < 27; ++i) {
+		std::memcpy(ctx->h_pblocks + (size_t)i * HCM_CHUNK_VOXELS, blocks[i], HCM_CHUNK_VOXELS * 2);
+		std::memcpy(ctx->h_pheights + (size_t)i * 256, heights[i], 256);
+	}
+	hipStream_t st = ctx->stream;
+	HCM_HIP(hipMemcpyAsync(ctx->d_pblocks, ctx->h_pblocks, 27 * HCM_CHUNK_VOXELS * 2, hipMemcpyHostToDevice, st));
+	HCM_HIP(hipMemcpyAsync(ctx->d_pheights, ctx->h_pheights, 27 * 256, hipMemcpyHostToDevice, st));
+	// light rebuild of the one job (slots 0..26 in neighbour-slot order)
+	LightParams lp;
+	std::memset(&lp, 0, sizeof(lp));
+	lp.blocks = ctx->d_pblocks;
+	lp.heights = ctx->d_pheights;
+	lp.light_info = ctx->d_light_info;
+	lp.slots = ctx->d_pnbr;
+	lp.n_slots = 27;
+	lp.n_jobs = 1;
+	lp.packed = ctx->d_ppacked;
+	lp.levels = ctx->d_plevels;
+	lp.nbr = ctx->d_pnbr;
+	lp.ctr = ctx->d_plctr;
+	lp.halo_light = ctx->d_phalo;
+	HCM_HIP(launch_reset(ctx->d_plctr, st));
+	HCM_HIP(launch_light_prep(lp, st));
+	HCM_HIP(launch_light(lp, 1, st));
+	// meshing over the rebuilt light
+	HCM_HIP(launch_reset(ctx->d_pctr, st));
+	MeshParams p = base_params(ctx);
+	p.blocks = ctx->d_pblocks;
+	p.light = ctx->d_plight;
+	p.halo_light = ctx->d_phalo;
+	p.nbr = ctx->d_pnbr;
+	p.n_jobs = 1;
+	p.job_base = 0;
+	p.persistent = 0;
+	p.pool_cap = kPlainPoolQuads;
+	p.ctr = ctx->d_pctr;
+	p.pool_verts = ctx->d_pverts;
+	p.pool_idx = ctx->d_pidx;
+	p.recs = ctx->d_precs;
+	p.rec_cap = kPlainRecs;
+	p.results = ctx->d_pres;
+	HCM_HIP(launch_mesh(p, (int)ctx->tuning.threads, 1, st));
+	hcm_job_result res;
+	HCM_HIP(hipMemcpyAsync(&res, ctx->d_pres, sizeof(res), hipMemcpyDeviceToHost, st));
+	std::vector<uint8_t> rows;
+	if (light_out) {
+		rows.resize(kLightJobBytes);
+		HCM_HIP(hipMemcpyAsync(rows.data(), ctx->d_phalo, kLightJobBytes, hipMemcpyDeviceToHost, st));
+	}
+	HCM_HIP(hipStreamSynchronize(st));
+	if (light_out)
+		for (uint32_t r = 0; r < 18 * 18; ++r) // row hz + 18 hy holds x = -1 .. 16
+			std::memcpy(light_out + 18 * r, rows.data() + r * kLightRowBytes, 18);
 	return fetch_job(st, res, ctx->d_precs, ctx->d_pverts, ctx->d_pidx, out);
 }
 

@@ -112,6 +112,7 @@ SIGNATURES = [
     ("hcm_last_hip_error", ctypes.c_int, []),
     ("hcm_device_ok", ctypes.c_int, []),
     ("hcm_mesh_chunk", ctypes.c_int, [_VP, _VP, _VP, ctypes.POINTER(HostResultC)]),
+    ("hcm_mesh_chunk_relight", ctypes.c_int, [_VP, _VP, _VP, ctypes.POINTER(HostResultC), _VP]),
     ("hcm_store_create", ctypes.c_int, [_VP, ctypes.c_uint32, ctypes.POINTER(_VP)]),
     ("hcm_store_destroy", None, [_VP]),
     ("hcm_store_upload", ctypes.c_int, [_VP, ctypes.c_uint32, ctypes.c_uint32, _VP, _VP]),
@@ -203,9 +204,9 @@ class Mesh:
         self.transparent = transparent
 
 
-def _ptrs27(arrs, dtype):
+def _ptrs27(arrs, dtype, size=4096):
     arrs = [np.ascontiguousarray(a, dtype=dtype).reshape(-1) for a in arrs]
-    assert len(arrs) == 27 and all(a.size == 4096 for a in arrs)
+    assert len(arrs) == 27 and all(a.size == size for a in arrs)
     p = (ctypes.c_void_p * 27)(*[a.ctypes.data for a in arrs])
     return p, arrs
 
@@ -269,6 +270,16 @@ class Context:
         pl, keep_l = _ptrs27(light27, np.uint8)
         check("hcm_mesh_chunk", lib().hcm_mesh_chunk(self._ptr, pb, pl, ctypes.byref(self._out.c)))
         return self._out.meshes()
+
+    def mesh_chunk_relight(self, blocks27, heights27):
+        """The whole mesh task: light rebuilt from 27 host chunks' blocks and sunlight heights (256 int8
+        each), then meshed. Returns (meshes, rebuilt light over [-1, 17)^3 as (18, 18, 18) u8 [y, z, x])."""
+        pb, keep_b = _ptrs27(blocks27, np.uint16)
+        ph, keep_h = _ptrs27(heights27, np.int8, size=256)
+        light = np.zeros(18 ** 3, np.uint8)
+        check("hcm_mesh_chunk_relight", lib().hcm_mesh_chunk_relight(self._ptr, pb, ph, ctypes.byref(self._out.c),
+                                                                      light.ctypes.data))
+        return self._out.meshes(), light.reshape(18, 18, 18)
 
 
 class Store:

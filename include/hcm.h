@@ -138,6 +138,13 @@ typedef struct hcm_host_result {
 int hcm_mesh_chunk(hcm_ctx *ctx, const uint16_t *const blocks[27], const uint8_t *const light[27],
                    hcm_host_result *out);
 
+/* The whole mesh task of ChunkTaskRunner<kMesh>::Run (client/src/ChunkMeshTask.cpp:41-122): rebuilds light
+ * from the 27 chunks' blocks and sunlight heights (int8[256] each, Chunk::m_sunlight_heights, index
+ * x + 16 z), then meshes over it. When light_out is not NULL it receives the rebuilt light over the 18^3
+ * cells [-1, 17)^3 the mesher reads, index (x+1) + 18 ((z+1) + 18 (y+1)). */
+int hcm_mesh_chunk_relight(hcm_ctx *ctx, const uint16_t *const blocks[27], const int8_t *const heights[27],
+                           hcm_host_result *out, uint8_t *light_out);
+
 /* ------------------------------------------------------------------------------------------------- */
 /* Device-resident chunk store and batched meshing (no counterpart in the reference, which meshes one
  * chunk per task on CPU threads; this is the batched form of the same call).                        */

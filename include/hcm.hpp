@@ -77,6 +77,23 @@ public:
 	// e.g. neighbour_chunks[i]->GetBlockData() at client/src/ChunkMeshTask.cpp:117-119. (A distinct name:
 	// arrays of pointers would otherwise bind to the callable overload above.)
 	std::vector<BlockMesh> GenerateChunks(const uint16_t *const blocks[27], const uint8_t *const light[27]) {
+		return Collect("hcm_mesh_chunk",
+		               [&](hcm_host_result *out) { return hcm_mesh_chunk(m_ctx, blocks, light, out); });
+	}
+
+	// The whole of ChunkTaskRunner<kMesh>::Run (client/src/ChunkMeshTask.cpp:41-122): light rebuilt from
+	// the 27 chunks' blocks and sunlight heights (Chunk::m_sunlight_heights, 16 x 16 int8 each, index
+	// x + 16 z), then meshed. heights[i] belongs to the chunk of blocks[i].
+	std::vector<BlockMesh> RunMeshTask(const uint16_t *const blocks[27], const int8_t *const heights[27]) {
+		return Collect("hcm_mesh_chunk_relight", [&](hcm_host_result *out) {
+			return hcm_mesh_chunk_relight(m_ctx, blocks, heights, out, nullptr);
+		});
+	}
+
+	hcm_ctx *context() const { return m_ctx; }
+
+private:
+	template <typename Call> std::vector<BlockMesh> Collect(const char *what, Call &&call) {
 		hcm_host_result out{};
 		for (;;) {
 			out.vertices = reinterpret_cast<uint32_t(*)[2]>(m_verts.data());
@@ -85,7 +102,7 @@ public:
 			out.index_capacity = (uint32_t)m_idx.size();
 			out.meshes = m_desc.data();
 			out.mesh_capacity = (uint32_t)m_desc.size();
-			int r = hcm_mesh_chunk(m_ctx, blocks, light, &out);
+			int r = call(&out);
 			if (r == HCM_ERR_CAPACITY) {
 				m_verts.resize(out.vertex_count);
 				m_idx.resize(out.index_count);
@@ -93,7 +110,7 @@ public:
 				continue;
 			}
 			if (r != HCM_OK)
-				throw Error("hcm_mesh_chunk", r);
+				throw Error(what, r);
 			break;
 		}
 		std::vector<BlockMesh> meshes(out.mesh_count);
@@ -108,9 +125,6 @@ public:
 		return meshes;
 	}
 
-	hcm_ctx *context() const { return m_ctx; }
-
-private:
 	static int slot_of(int x, int y, int z) {
 		auto l = [](int v) { return v < 0 ? 1 : (v >= 16 ? 0 : 2); };
 		return 9 * l(x) + 3 * l(y) + l(z);

@@ -177,3 +177,15 @@ def test_torch_levels_below_15_equal_oracle(oracle, golden):
         batch.close()
         store.close()
         ctx2.close()
+
+
+@pytest.mark.parametrize("name", SMALL)
+def test_plain_path_whole_mesh_task(ctx, golden, name):
+    """hcm_mesh_chunk_relight: light rebuild + meshing of one chunk from host arrays."""
+    blocks, heights, nbr = lset(name)
+    for j in range(0, nbr.shape[0], 3):
+        say(f"plain whole mesh task: {name} job {j}")
+        b27 = [blocks[s] for s in nbr[j]]
+        meshes, light = ctx.mesh_chunk_relight(b27, workloads.slice_heights(heights, nbr, j))
+        assert hashlib.sha256(light.tobytes()).digest() == golden[name + "_halo_digest"][j].tobytes()
+        assert meshdigest.digest(meshes) == golden[name + "_mesh_digest"][j].tobytes()
