@@ -438,6 +438,7 @@ int hcm::ctx_info(hcm_ctx *ctx, uint32_t threads, CtxInfo *out) {
 	out->n_cu = ctx->n_cu;
 	out->threads = threads ? threads : ctx->tuning.threads;
 	out->grid = (uint32_t)persistent_grid(ctx, out->threads);
+	out->light_info = ctx->d_light_info;
 	return HCM_OK;
 }
 

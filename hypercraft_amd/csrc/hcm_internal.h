@@ -118,7 +118,8 @@ struct CtxInfo {
 	DevReg reg;
 	int n_cu;
 	uint32_t threads;
-	uint32_t grid; // persistent grid for `threads`
+	uint32_t grid;              // persistent grid for `threads`
+	const uint8_t *light_info;  // [65536] for the light rebuild
 };
 int ctx_info(hcm_ctx *ctx, uint32_t threads, CtxInfo *out);
 

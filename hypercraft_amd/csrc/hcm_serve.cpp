@@ -135,7 +135,8 @@ struct Request {
 	Clock::time_point t_submit;
 	int32_t pos[3] = {0, 0, 0}; // chunk (load) or world voxel (edit)
 	std::vector<uint16_t> blocks;
-	std::vector<uint8_t> light;
+	std::vector<uint8_t> light;    // HCM_LIGHT_STORE loads
+	std::vector<int8_t> heights;   // HCM_LIGHT_REBUILD loads: sunlight heights
 	uint16_t block = 0;
 	uint8_t light1 = 0;
 	uint32_t slot = 0;             // load: the chunk's device slot
@@ -164,7 +165,12 @@ struct BatchSlot { // device and pinned buffers of one in-flight batch
 	VoxelEdit *h_edits = nullptr;
 	uint16_t *h_blocks = nullptr;
 	uint8_t *h_light = nullptr;
+	int8_t *h_heights = nullptr;
 	hcm_job_result *h_res = nullptr;
+	// light rebuild (HCM_LIGHT_REBUILD): the slots the jobs read, queues, rebuilt halo light
+	uint32_t *d_rslots = nullptr, *h_rslots = nullptr;
+	Counters *d_lctr = nullptr;
+	uint8_t *d_halo = nullptr;
 	hcm_mesh_rec *h_recs = nullptr;
 	Counters *h_ctr = nullptr;
 	hipEvent_t ev_up = nullptr, ev_k0 = nullptr, ev_k1 = nullptr, ev_done = nullptr;
@@ -193,6 +199,9 @@ struct hcm_server {
 	hipStream_t s_copy = nullptr, s_comp = nullptr;
 	uint16_t *d_blocks = nullptr;
 	uint8_t *d_light = nullptr;
+	int8_t *d_heights = nullptr;  // HCM_LIGHT_REBUILD: [slot][256] sunlight heights
+	uint16_t *d_packed = nullptr; // HCM_LIGHT_REBUILD: [slot][kLightPlanes][256]
+	uint32_t *d_levels = nullptr;
 	uint2 *d_verts = nullptr;
 	uint16_t *d_idx = nullptr;
 	std::vector<BatchSlot> slots;
@@ -228,6 +237,8 @@ struct hcm_server {
 	hcm_server_stats stats{};
 	std::vector<HistoryEntry> history;
 	std::thread thread;
+
+	bool relight() const { return cfg.light_mode == HCM_LIGHT_REBUILD; }
 
 	void fail(int code) {
 		error = code;
@@ -291,14 +302,26 @@ struct hcm_server {
 				edits.emplace(vkey, PendingEdit{ve, r->epoch});
 			else
 				e->second.e = ve;
-			// a voxel on a chunk border lies in the halo of the neighbours across that border
-			const int ox[2] = {0, lx == 0 ? -1 : (lx == 15 ? 1 : 0)};
-			const int oy[2] = {0, ly == 0 ? -1 : (ly == 15 ? 1 : 0)};
-			const int oz[2] = {0, lz == 0 ? -1 : (lz == 15 ? 1 : 0)};
-			for (int a = 0; a < (ox[1] ? 2 : 1); ++a)
-				for (int b = 0; b < (oy[1] ? 2 : 1); ++b)
-					for (int d = 0; d < (oz[1] ? 2 : 1); ++d)
-						mark_stale(cx + ox[a], cy + oy[b], cz + oz[d], r->targets);
+			if (relight()) {
+				// the neighbours within light distance 14: ChunkShouldRemesh (ChunkMeshTask.inl:9-15), the
+				// L1 distance of the voxel to the neighbour's bound [-1, 17)^3 below 15
+				auto dist = [](int v) { return v < -1 ? -1 - v : (v >= 17 ? v - 17 : 0); };
+				for (int s = 0; s < 27; ++s) {
+					int d[3];
+					slot_offset(s, d);
+					if (dist(lx - 16 * d[0]) + dist(ly - 16 * d[1]) + dist(lz - 16 * d[2]) < 15)
+						mark_stale(cx + d[0], cy + d[1], cz + d[2], r->targets);
+				}
+			} else {
+				// a voxel on a chunk border lies in the halo of the neighbours across that border
+				const int ox[2] = {0, lx == 0 ? -1 : (lx == 15 ? 1 : 0)};
+				const int oy[2] = {0, ly == 0 ? -1 : (ly == 15 ? 1 : 0)};
+				const int oz[2] = {0, lz == 0 ? -1 : (lz == 15 ? 1 : 0)};
+				for (int a = 0; a < (ox[1] ? 2 : 1); ++a)
+					for (int b = 0; b < (oy[1] ? 2 : 1); ++b)
+						for (int d = 0; d < (oz[1] ? 2 : 1); ++d)
+							mark_stale(cx + ox[a], cy + oy[b], cz + oz[d], r->targets);
+			}
 		}
 		waiting.push_back(std::move(r));
 	}
@@ -366,24 +389,31 @@ struct hcm_server {
 		for (uint32_t i = 0; i < n_up; ++i) {
 			Request *r = uploads[i].req;
 			std::memcpy(b.h_blocks + i * V, r->blocks.data(), V * 2);
-			std::memcpy(b.h_light + i * V, r->light.data(), V);
+			if (relight())
+				std::memcpy(b.h_heights + i * 256, r->heights.data(), 256);
+			else
+				std::memcpy(b.h_light + i * V, r->light.data(), V);
 			std::vector<uint16_t>().swap(r->blocks);
 			std::vector<uint8_t>().swap(r->light);
+			std::vector<int8_t>().swap(r->heights);
 		}
+		// a slot's data: blocks, then light bytes or sunlight heights
+		auto upload = [&](uint32_t i, hipStream_t st) -> hipError_t {
+			const uint32_t slot = uploads[i].slot;
+			hipError_t e = hipMemcpyAsync(d_blocks + slot * V, b.h_blocks + i * V, V * 2, hipMemcpyHostToDevice, st);
+			if (e == hipSuccess)
+				e = relight() ? hipMemcpyAsync(d_heights + slot * 256, b.h_heights + i * 256, 256, hipMemcpyHostToDevice, st)
+				              : hipMemcpyAsync(d_light + slot * V, b.h_light + i * V, V, hipMemcpyHostToDevice, st);
+			return e;
+		};
 		for (uint32_t i = 0; i < n_up; ++i)
-			if (!uploads[i].reload) {
-				const uint32_t slot = uploads[i].slot;
-				HCM_CHECK(hipMemcpyAsync(d_blocks + slot * V, b.h_blocks + i * V, V * 2, hipMemcpyHostToDevice, s_copy));
-				HCM_CHECK(hipMemcpyAsync(d_light + slot * V, b.h_light + i * V, V, hipMemcpyHostToDevice, s_copy));
-			}
+			if (!uploads[i].reload)
+				HCM_CHECK(upload(i, s_copy));
 		HCM_CHECK(hipEventRecord(b.ev_up, s_copy));
 		HCM_CHECK(hipStreamWaitEvent(s_comp, b.ev_up, 0));
 		for (uint32_t i = 0; i < n_up; ++i)
-			if (uploads[i].reload) {
-				const uint32_t slot = uploads[i].slot;
-				HCM_CHECK(hipMemcpyAsync(d_blocks + slot * V, b.h_blocks + i * V, V * 2, hipMemcpyHostToDevice, s_comp));
-				HCM_CHECK(hipMemcpyAsync(d_light + slot * V, b.h_light + i * V, V, hipMemcpyHostToDevice, s_comp));
-			}
+			if (uploads[i].reload)
+				HCM_CHECK(upload(i, s_comp));
 		uploads.erase(uploads.begin(), uploads.begin() + n_up);
 
 		uint32_t k = 0;
@@ -410,6 +440,30 @@ struct hcm_server {
 				}
 			}
 			HCM_CHECK(hipMemcpyAsync(b.d_nbr, b.h_nbr, b.jobs.size() * 27 * 4, hipMemcpyHostToDevice, s_comp));
+			HCM_CHECK(hipEventRecord(b.ev_k0, s_comp));
+			if (relight()) { // rebuild the jobs' light from the slots they read (ChunkMeshTask.cpp:48-111)
+				std::vector<uint32_t> rs(b.h_nbr, b.h_nbr + 27 * b.jobs.size());
+				std::sort(rs.begin(), rs.end());
+				rs.erase(std::unique(rs.begin(), rs.end()), rs.end());
+				std::memcpy(b.h_rslots, rs.data(), rs.size() * 4);
+				HCM_CHECK(hipMemcpyAsync(b.d_rslots, b.h_rslots, rs.size() * 4, hipMemcpyHostToDevice, s_comp));
+				LightParams lp;
+				std::memset(&lp, 0, sizeof(lp));
+				lp.blocks = d_blocks;
+				lp.heights = d_heights;
+				lp.light_info = info.light_info;
+				lp.slots = b.d_rslots;
+				lp.n_slots = (uint32_t)rs.size();
+				lp.n_jobs = (uint32_t)b.jobs.size();
+				lp.packed = d_packed;
+				lp.levels = d_levels;
+				lp.nbr = b.d_nbr;
+				lp.ctr = b.d_lctr;
+				lp.halo_light = b.d_halo;
+				HCM_CHECK(launch_reset(b.d_lctr, s_comp));
+				HCM_CHECK(launch_light_prep(lp, s_comp));
+				HCM_CHECK(launch_light(lp, std::max(1, std::min(info.n_cu * 2, (int)b.jobs.size())), s_comp));
+			}
 			HCM_CHECK(launch_reset(b.d_ctr, s_comp));
 			MeshParams p;
 			std::memset(&p, 0, sizeof(p));
@@ -426,8 +480,8 @@ struct hcm_server {
 			p.recs = b.d_recs;
 			p.rec_cap = rec_cap();
 			p.results = b.d_res;
+			p.halo_light = relight() ? b.d_halo : nullptr;
 			const int grid = (int)std::max<uint32_t>(1, std::min<uint32_t>(info.grid, (uint32_t)b.jobs.size()));
-			HCM_CHECK(hipEventRecord(b.ev_k0, s_comp));
 			HCM_CHECK(launch_mesh(p, (int)info.threads, grid, s_comp));
 			HCM_CHECK(hipEventRecord(b.ev_k1, s_comp));
 			HCM_CHECK(hipMemcpyAsync(b.h_res, b.d_res, b.jobs.size() * sizeof(hcm_job_result), hipMemcpyDeviceToHost,
@@ -627,18 +681,20 @@ struct hcm_server {
 
 static void free_server(hcm_server *s) {
 	for (auto &b : s->slots) {
-		for (void *p : {(void *)b.d_nbr, (void *)b.d_edits, (void *)b.d_ctr, (void *)b.d_res, (void *)b.d_recs})
+		for (void *p : {(void *)b.d_nbr, (void *)b.d_edits, (void *)b.d_ctr, (void *)b.d_res, (void *)b.d_recs,
+		                (void *)b.d_rslots, (void *)b.d_lctr, (void *)b.d_halo})
 			if (p)
 				(void)hipFree(p);
 		for (void *p : {(void *)b.h_nbr, (void *)b.h_edits, (void *)b.h_blocks, (void *)b.h_light, (void *)b.h_res,
-		                (void *)b.h_recs, (void *)b.h_ctr})
+		                (void *)b.h_recs, (void *)b.h_ctr, (void *)b.h_heights, (void *)b.h_rslots})
 			if (p)
 				(void)hipHostFree(p);
 		for (hipEvent_t e : {b.ev_up, b.ev_k0, b.ev_k1, b.ev_done})
 			if (e)
 				(void)hipEventDestroy(e);
 	}
-	for (void *p : {(void *)s->d_blocks, (void *)s->d_light, (void *)s->d_verts, (void *)s->d_idx})
+	for (void *p : {(void *)s->d_blocks, (void *)s->d_light, (void *)s->d_verts, (void *)s->d_idx, (void *)s->d_heights,
+	                (void *)s->d_packed, (void *)s->d_levels})
 		if (p)
 			(void)hipFree(p);
 	if (s->s_copy)
@@ -665,6 +721,11 @@ static int server_init(hcm_server *s) {
 	HCM_CHECK(hipStreamCreateWithFlags(&s->s_comp, hipStreamNonBlocking));
 	HCM_CHECK(dalloc(&s->d_blocks, (size_t)c.max_chunks * HCM_CHUNK_VOXELS));
 	HCM_CHECK(dalloc(&s->d_light, (size_t)c.max_chunks * HCM_CHUNK_VOXELS));
+	if (s->relight()) {
+		HCM_CHECK(dalloc(&s->d_heights, (size_t)c.max_chunks * 256));
+		HCM_CHECK(dalloc(&s->d_packed, (size_t)c.max_chunks * kLightPlanes * 256));
+		HCM_CHECK(dalloc(&s->d_levels, (size_t)c.max_chunks));
+	}
 	HCM_CHECK(dalloc(&s->d_verts, (size_t)c.pool_quads * 4));
 	HCM_CHECK(dalloc(&s->d_idx, (size_t)c.pool_quads * 6));
 	s->pool = new RangeAllocator(c.pool_quads);
@@ -679,6 +740,13 @@ static int server_init(hcm_server *s) {
 		HCM_CHECK(halloc(&b.h_edits, kMaxEdits));
 		HCM_CHECK(halloc(&b.h_blocks, (size_t)kMaxUploads * HCM_CHUNK_VOXELS));
 		HCM_CHECK(halloc(&b.h_light, (size_t)kMaxUploads * HCM_CHUNK_VOXELS));
+		if (s->relight()) {
+			HCM_CHECK(halloc(&b.h_heights, (size_t)kMaxUploads * 256));
+			HCM_CHECK(dalloc(&b.d_rslots, (size_t)c.max_batch * 27));
+			HCM_CHECK(halloc(&b.h_rslots, (size_t)c.max_batch * 27));
+			HCM_CHECK(dalloc(&b.d_lctr, 1));
+			HCM_CHECK(dalloc(&b.d_halo, (size_t)c.max_batch * kLightJobBytes));
+		}
 		HCM_CHECK(halloc(&b.h_res, c.max_batch));
 		HCM_CHECK(halloc(&b.h_recs, s->rec_cap()));
 		HCM_CHECK(halloc(&b.h_ctr, 1));
@@ -696,7 +764,7 @@ extern "C" int hcm_server_create(hcm_ctx *ctx, const hcm_server_config *cfg, hcm
 	*out = nullptr;
 	if (!ctx || !cfg || cfg->max_chunks == 0 || cfg->max_chunks > (1u << 20) || cfg->max_batch == 0 ||
 	    cfg->max_batch > (1u << 20) || cfg->max_inflight == 0 || cfg->max_inflight > 16 || cfg->pool_quads == 0 ||
-	    cfg->pool_quads >= (1ull << 32))
+	    cfg->pool_quads >= (1ull << 32) || (cfg->light_mode != HCM_LIGHT_STORE && cfg->light_mode != HCM_LIGHT_REBUILD))
 		return HCM_ERR_INVALID;
 	hcm_server *s = new hcm_server;
 	s->ctx = ctx;
@@ -726,10 +794,8 @@ extern "C" void hcm_server_destroy(hcm_server *s) {
 	free_server(s);
 }
 
-extern "C" int hcm_server_load(hcm_server *s, int32_t cx, int32_t cy, int32_t cz, const uint16_t *blocks,
-                               const uint8_t *light, uint64_t tag) {
-	if (!s || !blocks || !light)
-		return HCM_ERR_INVALID;
+static int submit_load(hcm_server *s, int32_t cx, int32_t cy, int32_t cz, const uint16_t *blocks,
+                       const uint8_t *light, const int8_t *heights, uint64_t tag) {
 	if (cx < -(1 << 20) || cx >= (1 << 20) || cy < -(1 << 20) || cy >= (1 << 20) || cz < -(1 << 20) || cz >= (1 << 20))
 		return HCM_ERR_INVALID;
 	auto r = std::make_unique<Request>();
@@ -737,7 +803,10 @@ extern "C" int hcm_server_load(hcm_server *s, int32_t cx, int32_t cy, int32_t cz
 	r->kind = HCM_REQ_LOAD;
 	r->pos[0] = cx, r->pos[1] = cy, r->pos[2] = cz;
 	r->blocks.assign(blocks, blocks + HCM_CHUNK_VOXELS);
-	r->light.assign(light, light + HCM_CHUNK_VOXELS);
+	if (light)
+		r->light.assign(light, light + HCM_CHUNK_VOXELS);
+	if (heights)
+		r->heights.assign(heights, heights + 256);
 	r->t_submit = Clock::now();
 	{
 		std::lock_guard<std::mutex> lk(s->in_mu);
@@ -756,6 +825,20 @@ extern "C" int hcm_server_load(hcm_server *s, int32_t cx, int32_t cy, int32_t cz
 	}
 	s->in_cv.notify_one();
 	return HCM_OK;
+}
+
+extern "C" int hcm_server_load(hcm_server *s, int32_t cx, int32_t cy, int32_t cz, const uint16_t *blocks,
+                               const uint8_t *light, uint64_t tag) {
+	if (!s || !blocks || !light || s->relight())
+		return HCM_ERR_INVALID;
+	return submit_load(s, cx, cy, cz, blocks, light, nullptr, tag);
+}
+
+extern "C" int hcm_server_load_sunlit(hcm_server *s, int32_t cx, int32_t cy, int32_t cz, const uint16_t *blocks,
+                                      const int8_t *heights, uint64_t tag) {
+	if (!s || !blocks || !heights || !s->relight())
+		return HCM_ERR_INVALID;
+	return submit_load(s, cx, cy, cz, blocks, nullptr, heights, tag);
 }
 
 extern "C" int hcm_server_edit(hcm_server *s, int32_t x, int32_t y, int32_t z, uint16_t block, uint8_t light,

@@ -82,7 +82,7 @@ class TuningC(ctypes.Structure):
 class ServerConfigC(ctypes.Structure):
     _fields_ = [("max_chunks", ctypes.c_uint32), ("max_batch", ctypes.c_uint32), ("max_inflight", ctypes.c_uint32),
                 ("threads", ctypes.c_uint32), ("pool_quads", ctypes.c_uint64), ("keep_history", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32 * 3)]
+                ("light_mode", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 2)]
 
 
 class CompletionC(ctypes.Structure):
@@ -134,6 +134,8 @@ SIGNATURES = [
     ("hcm_server_destroy", None, [_VP]),
     ("hcm_server_load", ctypes.c_int, [_VP, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _VP, _VP,
                                        ctypes.c_uint64]),
+    ("hcm_server_load_sunlit", ctypes.c_int, [_VP, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _VP, _VP,
+                                              ctypes.c_uint64]),
     ("hcm_server_edit", ctypes.c_int, [_VP, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_uint16,
                                        ctypes.c_uint8, ctypes.c_uint64]),
     ("hcm_server_poll", ctypes.c_int, [_VP, ctypes.POINTER(CompletionC), ctypes.c_uint32,
@@ -393,10 +395,10 @@ class Server:
     """Continuous batching of chunk loads and voxel edits (hcm_server)."""
 
     def __init__(self, ctx, max_chunks, max_batch=4096, max_inflight=2, threads=0, pool_quads=1 << 24,
-                 keep_history=False):
+                 keep_history=False, light_mode=LIGHT_STORE):
         self.ctx = ctx
         cfg = ServerConfigC(max_chunks, max_batch, max_inflight, threads, pool_quads, int(keep_history),
-                            (ctypes.c_uint32 * 3)(0, 0, 0))
+                            light_mode, (ctypes.c_uint32 * 2)(0, 0))
         self._ptr = ctypes.c_void_p()
         check("hcm_server_create", lib().hcm_server_create(ctx.ptr, ctypes.byref(cfg), ctypes.byref(self._ptr)))
         self._out = _HostOut()
@@ -406,6 +408,13 @@ class Server:
         b = np.ascontiguousarray(blocks, np.uint16).reshape(4096)
         l = np.ascontiguousarray(light, np.uint8).reshape(4096)
         check("hcm_server_load", lib().hcm_server_load(self._ptr, cx, cy, cz, b.ctypes.data, l.ctypes.data, tag))
+
+    def load_sunlit(self, cx, cy, cz, blocks, heights, tag=0):
+        """A LIGHT_REBUILD server's load: blocks (4096 u16) and sunlight heights (256 int8)."""
+        b = np.ascontiguousarray(blocks, np.uint16).reshape(4096)
+        h = np.ascontiguousarray(heights, np.int8).reshape(256)
+        check("hcm_server_load_sunlit", lib().hcm_server_load_sunlit(self._ptr, cx, cy, cz, b.ctypes.data,
+                                                                      h.ctypes.data, tag))
 
     def edit(self, x, y, z, block, light, tag=0):
         check("hcm_server_edit", lib().hcm_server_edit(self._ptr, x, y, z, block, light, tag))

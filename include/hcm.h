@@ -240,7 +240,11 @@ typedef struct hcm_server_config {
 	uint32_t threads;      /* workgroup size of the meshing kernel (0 = context tuning) */
 	uint64_t pool_quads;   /* vertex pool for the meshes kept on the device */
 	uint32_t keep_history; /* 1: keep a host copy of every batch's meshes (tests) */
-	uint32_t reserved[3];
+	/* HCM_LIGHT_STORE: loads and edits carry light bytes. HCM_LIGHT_REBUILD: loads carry sunlight heights
+	 * (hcm_server_load_sunlit) and every batch rebuilds its chunks' light before meshing, as the
+	 * reference's mesh task does (client/src/ChunkMeshTask.cpp:48-111). */
+	uint32_t light_mode;
+	uint32_t reserved[2];
 } hcm_server_config;
 
 #define HCM_REQ_LOAD 1u
@@ -266,8 +270,16 @@ void hcm_server_destroy(hcm_server *srv);
  * loaded neighbour whose 26 neighbours are then all present are (re)meshed. HCM_ERR_BUSY when no slot. */
 int hcm_server_load(hcm_server *srv, int32_t cx, int32_t cy, int32_t cz, const uint16_t *blocks,
                     const uint8_t *light, uint64_t tag);
-/* Sets the block and light byte of world voxel (x, y, z) in a loaded chunk and remeshes the chunks whose
- * 18^3 halo holds the voxel (the edited chunk and 0..7 neighbours). HCM_ERR_INVALID if not loaded. */
+/* HCM_LIGHT_REBUILD servers: loads chunk (cx, cy, cz) with its 16 x 16 sunlight heights (int8, index
+ * x + 16 z, Chunk::m_sunlight_heights, client/include/client/Chunk.hpp:96) instead of light bytes.
+ * HCM_ERR_INVALID on a HCM_LIGHT_STORE server, as hcm_server_load is on a HCM_LIGHT_REBUILD one. */
+int hcm_server_load_sunlit(hcm_server *srv, int32_t cx, int32_t cy, int32_t cz, const uint16_t *blocks,
+                           const int8_t *heights, uint64_t tag);
+/* Sets the block (and, with HCM_LIGHT_STORE, the light byte) of world voxel (x, y, z) in a loaded chunk
+ * and remeshes the chunks whose mesh can change: with HCM_LIGHT_STORE those whose 18^3 halo holds the
+ * voxel (the edited chunk and 0..7 neighbours); with HCM_LIGHT_REBUILD those within light distance 14 of
+ * it, the reference's rule (ChunkSetBlockTask.cpp:100-107, ChunkMeshTask.inl:9-15). HCM_ERR_INVALID if
+ * the chunk is not loaded. */
 int hcm_server_edit(hcm_server *srv, int32_t x, int32_t y, int32_t z, uint16_t block, uint8_t light, uint64_t tag);
 /* Completed requests since the last poll (up to cap); *n receives the count. */
 int hcm_server_poll(hcm_server *srv, hcm_completion *out, uint32_t cap, uint32_t *n);

@@ -5,6 +5,7 @@ mesh of that chunk meshed alone, by the CPU oracle, on the world as it stood at 
 (the number of requests applied before its batch) -- bit-exact -- and that every request completes
 once, after every chunk it made stale holds a mesh at or after its epoch.
 """
+import os
 import sys
 
 import numpy as np
@@ -12,6 +13,7 @@ import pytest
 
 import cases
 import meshdigest
+from conftest import GOLDEN
 from hypercraft_amd import hcm, workloads
 
 pytestmark = pytest.mark.gpu
@@ -61,10 +63,10 @@ def inner():
     return [p for p in grid() if all(0 < p[a] < DIMS[a] - 1 for a in range(3))]
 
 
-def check_history(srv, oracle, log):
+def check_history(srv, oracle, log, world=None):
     """Replays the request log and compares every recorded mesh with the oracle alone at its epoch."""
     hist = sorted(srv.history(), key=lambda h: h[1])
-    world = World()
+    world = world if world is not None else World()
     applied = 0
     for pos, epoch, meshes in hist:
         while applied < epoch:
@@ -226,6 +228,82 @@ def test_server_pool_pressure(ctx, oracle):
         assert st["pool_used_quads"] + st["pool_free_quads"] == 3 << 16
         live = sum(sum(len(m.vertices) // 4 for m in srv.chunk_mesh(*p)) for p in inner())
         assert st["pool_used_quads"] == live  # superseded meshes and unused arena tails went back to the pool
+    finally:
+        srv.close()
+
+
+class LitWorld(World):
+    """Host world of a LIGHT_REBUILD server: chunk position -> (blocks, sunlight heights); a chunk is
+    meshed alone by the oracle's light rebuild followed by the oracle's mesher (ChunkMeshTask.cpp:41-122)."""
+
+    def __init__(self, levels):
+        super().__init__()
+        self.levels = levels
+
+    def edit(self, x, y, z, block, light):
+        c = (x // 16, y // 16, z // 16)
+        self.chunks[c][0][(x % 16) + 16 * ((z % 16) + 16 * (y % 16))] = block
+
+    def mesh_alone(self, oracle, pos):
+        nb = [self.chunks[tuple(int(v) for v in np.add(pos, d))] for d in workloads.SLOT_OFFSETS]
+        b27 = [b for b, _ in nb]
+        win, _ = oracle.light_rebuild(self.levels, b27, [h for _, h in nb])
+        return oracle.mesh(b27, cases.light27_from_window(win))
+
+
+def light_chunks(world, x, y, z):
+    """Meshable chunks within light distance 14 of world voxel (x, y, z) (ChunkMeshTask.inl:9-15)."""
+    def dist(v):
+        return -1 - v if v < -1 else (v - 17 if v >= 17 else 0)
+
+    c = (x // 16, y // 16, z // 16)
+    out = set()
+    for d in workloads.SLOT_OFFSETS:
+        n = tuple(int(c[a] + d[a]) for a in range(3))
+        if world.meshable(n) and sum(dist((x, y, z)[a] - 16 * n[a]) for a in range(3)) < 15:
+            out.add(n)
+    return out
+
+
+@pytest.mark.parametrize("max_batch,max_inflight", [(1, 1), (8, 3)])
+def test_server_relight_matches_mesh_task_alone(ctx, oracle, max_batch, max_inflight):
+    """A LIGHT_REBUILD server: loads carry sunlight heights, every batch rebuilds light before meshing,
+    edits remesh the chunks within light distance 14; every mesh equals the whole mesh task run alone."""
+    levels = np.load(os.path.join(GOLDEN, "light.npz"))["light_levels"]
+    rng = np.random.default_rng(300 + max_batch)
+    srv = hcm.Server(ctx, max_chunks=64, max_batch=max_batch, max_inflight=max_inflight, pool_quads=1 << 22,
+                     keep_history=True, light_mode=hcm.LIGHT_REBUILD)
+    try:
+        world, log = LitWorld(levels), []
+        say(f"serve relight: loads, max_batch {max_batch}, max_inflight {max_inflight}")
+        for i, pos in enumerate(grid()):
+            b, _ = terrain(pos)
+            b = workloads.place_glowstone(b.reshape(1, -1), 400 + i, 0.003).reshape(-1)
+            h = cases.random_heights(500 + i, 1)[0]
+            srv.load_sunlit(*pos, b, h, tag=len(log))
+            world.chunks[pos] = (b.copy(), h.copy())
+            log.append(("load", pos, b, h))
+        say("serve relight: 120 edits")
+        palette = np.array([0, 0, 1, 3, 9, 0x18, 0x10, 0x0E], np.uint16)  # air, stone, dirt, glass, glowstone, leaves, water
+        for k, (x, y, z, _, _) in enumerate(_edits(rng, 120)):
+            blk = int(palette[rng.integers(len(palette))])
+            srv.edit(x, y, z, blk, 0, tag=len(log))
+            world.edit(x, y, z, blk, 0)
+            log.append(("edit", x, y, z, blk, 0))
+            if k % 40 == 39:
+                srv.drain()
+        srv.drain()
+        done = srv.poll()
+        check_completions(done, len(log), log)
+        for c in done:
+            if c["kind"] == hcm.REQ_EDIT:
+                _, x, y, z = log[c["tag"]][:4]
+                assert c["remeshed"] == len(light_chunks(world, x, y, z))
+        n = check_history(srv, oracle, log, LitWorld(levels))
+        say(f"serve relight: {n} recorded meshes equal the oracle's mesh task alone at their epochs")
+        check_final(srv, oracle, world)
+        with pytest.raises(hcm.HcmError):
+            srv.load(0, 0, 0, *terrain((0, 0, 0)))  # light bytes are not what this server takes
     finally:
         srv.close()
 
