@@ -174,13 +174,17 @@ def mixed_leg(ctx, hcm, workloads, meshdigest, golden):
             "graph_vs_per_chunk": round(plain_ms / graph_ms, 2)}
 
 
-def serve_leg(ctx, hcm, workloads):
-    """Replays the spiral load + edit trace against the server in real time; latency per request kind."""
-    log("serving: generating the spiral trace")
-    tr = workloads.spiral_trace()
+def serve_leg(ctx, hcm, workloads, tr, relight=False):
+    """Replays the spiral load + edit trace against the server in real time; latency per request kind.
+    With relight the server rebuilds light before meshing: loads carry sunlight heights (the first lit
+    voxel of each terrain column) and edits remesh the chunks within light distance 14."""
     coords, blocks, light, events = tr["coords"], tr["blocks"], tr["light"], tr["events"]
     pool_quads = (16 << 30) // 44  # a 16 GB vertex pool (44 B per quad)
-    srv = hcm.Server(ctx, max_chunks=coords.shape[0], max_batch=4096, max_inflight=2, pool_quads=pool_quads)
+    srv = hcm.Server(ctx, max_chunks=coords.shape[0], max_batch=4096, max_inflight=2, pool_quads=pool_quads,
+                     light_mode=hcm.LIGHT_REBUILD if relight else hcm.LIGHT_STORE)
+    if relight:  # the terrain's light is sun 15 at and above the surface: heights = 16 - lit voxels per column
+        lit = ((light.reshape(-1, 16, 16, 16) & 15) == 15).sum(1)  # [chunk, z, x]
+        heights = (16 - lit).astype(np.int8).reshape(-1, 256)
     kinds = {}
     lag = 0.0
     log(f"serving: {len(events)} requests over {events[-1][0]:.2f} s")
@@ -192,7 +196,10 @@ def serve_leg(ctx, hcm, workloads):
         lag = max(lag, now - t)
         if kind == "load":
             c = coords[payload]
-            srv.load(int(c[0]), int(c[1]), int(c[2]), blocks[payload], light[payload], tag)
+            if relight:
+                srv.load_sunlit(int(c[0]), int(c[1]), int(c[2]), blocks[payload], heights[payload], tag)
+            else:
+                srv.load(int(c[0]), int(c[1]), int(c[2]), blocks[payload], light[payload], tag)
         else:
             srv.edit(*payload, tag)
         kinds[tag] = kind
@@ -345,7 +352,11 @@ def main():
         res["mixed_batches"] = mixed_leg(ctx, hcm, workloads, meshdigest, golden)
     if not args.no_serve:
         log("serving trace")
-        res["serving"] = serve_leg(ctx, hcm, workloads)
+        log("serving: generating the spiral trace")
+        trace = workloads.spiral_trace()
+        res["serving"] = serve_leg(ctx, hcm, workloads, trace)
+        log("serving trace with light rebuild")
+        res["serving_relight"] = serve_leg(ctx, hcm, workloads, trace, relight=True)
     if not args.no_relight:
         log("light rebuild + meshing")
         res["relight"] = relight_leg(ctx, hcm, workloads, meshdigest, 20, min(16, os.cpu_count() or 1),
