@@ -147,13 +147,15 @@ int build_tables(const hcm_registry *r, DeviceTables *t) {
 			}
 		t->pid_of[65536 + id] = v;
 	}
-	// the light rebuild's view of a block: Block::GetLightLevel and GetIndirectLightPass (Block.hpp:70-71)
+	// the light rebuild's view of a block: Block::GetLightLevel, GetIndirectLightPass and
+	// GetVerticalLightPass (Block.hpp:69-71)
 	t->light_info.resize(65536);
 	for (uint32_t d = 0; d < 65536; ++d) {
 		const uint32_t lv = r->light_level ? r->light_level[d] : 0u;
 		if (lv > 15)
 			return HCM_ERR_INVALID;
-		t->light_info[d] = uint8_t(lv | (r->rows[r->row_of[d]].transparency != 0 ? kLightInfoPass : 0u));
+		const uint32_t tr = r->rows[r->row_of[d]].transparency;
+		t->light_info[d] = uint8_t(lv | (tr != 0 ? kLightInfoPass : 0u) | (tr == 2 ? kLightInfoVertical : 0u));
 	}
 	t->meshes.assign(r->meshes, r->meshes + r->n_meshes);
 	t->tex_flags.assign(r->tex_flags, r->tex_flags + r->n_tex);
@@ -182,6 +184,7 @@ struct hcm_ctx {
 	hcm_custom_mesh *d_meshes = nullptr;
 	uint8_t *d_tex_flags = nullptr;
 	uint8_t *d_light_info = nullptr;
+	std::vector<uint8_t> light_info; // host copy
 	// batched pool
 	uint64_t pool_quads = 0;
 	uint2 *d_verts = nullptr;
@@ -318,6 +321,7 @@ static int create_impl(hcm_ctx *c, const hcm_registry *registry, uint64_t pool_q
 	HCM_HIP(dmalloc(&c->d_tex_flags, t.tex_flags.size()));
 	HCM_HIP(dmalloc(&c->d_light_info, t.light_info.size()));
 	HCM_HIP(hipMemcpy(c->d_light_info, t.light_info.data(), t.light_info.size(), hipMemcpyHostToDevice));
+	c->light_info = t.light_info;
 	HCM_HIP(hipMemcpy(c->d_pid_of, t.pid_of.data(), t.pid_of.size(), hipMemcpyHostToDevice));
 	HCM_HIP(hipMemcpy(c->d_rows, t.rows.data(), t.rows.size() * sizeof(uint2), hipMemcpyHostToDevice));
 	HCM_HIP(hipMemcpy(c->d_pid_vox, t.pid_vox.data(), t.pid_vox.size() * 4, hipMemcpyHostToDevice));
@@ -439,6 +443,7 @@ int hcm::ctx_info(hcm_ctx *ctx, uint32_t threads, CtxInfo *out) {
 	out->threads = threads ? threads : ctx->tuning.threads;
 	out->grid = (uint32_t)persistent_grid(ctx, out->threads);
 	out->light_info = ctx->d_light_info;
+	out->light_info_host = ctx->light_info.data();
 	return HCM_OK;
 }
 

@@ -82,7 +82,8 @@ struct MeshParams {
 constexpr int kLightPlanes = 6;
 constexpr uint32_t kLightRowBytes = 20;                        // one halo x-row of rebuilt light (18 used)
 constexpr uint32_t kLightJobBytes = 18 * 18 * kLightRowBytes; // rows hz + 18 hy
-constexpr uint8_t kLightInfoPass = 0x10; // light_info: level | kLightInfoPass when the block passes light
+constexpr uint8_t kLightInfoPass = 0x10;     // light_info: level | kLightInfoPass when the block passes light
+constexpr uint8_t kLightInfoVertical = 0x20; // ... | kLightInfoVertical when it passes sunlight straight down
 
 struct LightParams {
 	const uint16_t *blocks;    // store: slot * 4096 + local
@@ -98,7 +99,8 @@ struct LightParams {
 	uint8_t *halo_light;       // [n_jobs][kLightJobBytes]
 };
 
-// One voxel edit of a device chunk store: slot, chunk-local index, block, light.
+// One voxel edit of a device chunk store: slot, chunk-local index, block, light. local >= 4096 sets the
+// slot's sunlight height local - 4096 (index x + 16 z) to the low byte of block instead.
 struct VoxelEdit {
 	uint32_t slot, local, block, light;
 };
@@ -106,7 +108,8 @@ struct VoxelEdit {
 // Launchers (hcm_kernels.hip)
 hipError_t launch_reset(Counters *ctr, hipStream_t stream);
 hipError_t launch_mesh(const MeshParams &p, int threads, int grid, hipStream_t stream);
-hipError_t launch_edits(uint16_t *blocks, uint8_t *light, const VoxelEdit *edits, uint32_t n, hipStream_t stream);
+hipError_t launch_edits(uint16_t *blocks, uint8_t *light, int8_t *heights, const VoxelEdit *edits, uint32_t n,
+                        hipStream_t stream);
 int mesh_lds_bytes(int threads);
 // Launchers (hcm_light.hip)
 hipError_t launch_light_prep(const LightParams &p, hipStream_t stream);
@@ -120,6 +123,7 @@ struct CtxInfo {
 	uint32_t threads;
 	uint32_t grid;              // persistent grid for `threads`
 	const uint8_t *light_info;  // [65536] for the light rebuild
+	const uint8_t *light_info_host; // the same table in host memory
 };
 int ctx_info(hcm_ctx *ctx, uint32_t threads, CtxInfo *out);
 

@@ -1409,21 +1409,27 @@ __global__ void reset_kernel(Counters *ctr) {
 }
 
 // Applies voxel edits in order of their index; edits to one voxel within a launch are resolved by the
-// host, which keeps only the last one.
-__global__ void edit_kernel(uint16_t *blocks, uint8_t *light, const VoxelEdit *edits, uint32_t n) {
+// host, which keeps only the last one. An edit with local >= 4096 sets sunlight height local - 4096 of
+// its slot to the low byte of `block` (heights may be null when no such edit is sent).
+__global__ void edit_kernel(uint16_t *blocks, uint8_t *light, int8_t *heights, const VoxelEdit *edits, uint32_t n) {
 	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i < n) {
 		const VoxelEdit e = edits[i];
+		if (e.local >= HCM_CHUNK_VOXELS) {
+			heights[(size_t)e.slot * 256 + (e.local - HCM_CHUNK_VOXELS)] = int8_t(uint8_t(e.block));
+			return;
+		}
 		const size_t g = (size_t)e.slot * HCM_CHUNK_VOXELS + e.local;
 		blocks[g] = uint16_t(e.block);
 		light[g] = uint8_t(e.light);
 	}
 }
 
-hipError_t launch_edits(uint16_t *blocks, uint8_t *light, const VoxelEdit *edits, uint32_t n, hipStream_t stream) {
+hipError_t launch_edits(uint16_t *blocks, uint8_t *light, int8_t *heights, const VoxelEdit *edits, uint32_t n,
+                        hipStream_t stream) {
 	if (n == 0)
 		return hipSuccess;
-	hipLaunchKernelGGL(edit_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, blocks, light, edits, n);
+	hipLaunchKernelGGL(edit_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, blocks, light, heights, edits, n);
 	return hipGetLastError();
 }
 

@@ -127,6 +127,9 @@ struct Chunk {
 	uint64_t meshed_epoch = 0;      // epoch of the batch that produced the current meshes
 	Range mesh;                     // their pool range
 	std::vector<hcm_mesh_rec> recs; // their records (absolute first_quad)
+	// HCM_LIGHT_REBUILD: host copies of the blocks and sunlight heights, for the sunlight flood of edits
+	std::vector<uint16_t> blocks;
+	std::vector<int8_t> heights;
 };
 
 struct Request {
@@ -221,7 +224,7 @@ struct hcm_server {
 	std::deque<uint64_t> stale_fifo;
 	std::vector<std::unique_ptr<Request>> waiting;
 	std::deque<Upload> uploads;
-	std::map<uint64_t, PendingEdit> edits; // (slot << 12 | local) -> last edit of that voxel
+	std::map<uint64_t, PendingEdit> edits; // (slot << 13 | local) -> last edit of that voxel or height
 	uint64_t epoch = 0, batches = 0, applied_done = 0;
 	RangeAllocator *pool = nullptr;
 	double quads_per_chunk = 512.0; // running estimate that sizes arenas
@@ -278,11 +281,15 @@ struct hcm_server {
 			if (!reload) {
 				Chunk c;
 				c.slot = slot;
-				world.emplace(key, c);
+				it = world.emplace(key, c).first;
+			}
+			if (relight()) {
+				it->second.blocks = r->blocks;
+				it->second.heights = r->heights;
 			}
 			// pending edits of the chunk are superseded by the new data
-			auto e = edits.lower_bound(uint64_t(slot) << 12);
-			while (e != edits.end() && (e->first >> 12) == slot)
+			auto e = edits.lower_bound(uint64_t(slot) << 13);
+			while (e != edits.end() && (e->first >> 13) == slot)
 				e = edits.erase(e);
 			uploads.push_back(Upload{slot, reload, r.get()});
 			for (int s = 0; s < 27; ++s) {
@@ -293,15 +300,43 @@ struct hcm_server {
 		} else {
 			const int32_t cx = floor_div16(r->pos[0]), cy = floor_div16(r->pos[1]), cz = floor_div16(r->pos[2]);
 			const int32_t lx = r->pos[0] - 16 * cx, ly = r->pos[1] - 16 * cy, lz = r->pos[2] - 16 * cz;
-			const Chunk &c = world.at(chunk_key(cx, cy, cz));
+			Chunk &c = world.at(chunk_key(cx, cy, cz));
 			const uint32_t local = uint32_t(lx) + 16u * (uint32_t(lz) + 16u * uint32_t(ly));
-			const uint64_t vkey = (uint64_t(c.slot) << 12) | local;
-			auto e = edits.find(vkey);
-			const VoxelEdit ve{c.slot, local, r->block, r->light1};
-			if (e == edits.end())
-				edits.emplace(vkey, PendingEdit{ve, r->epoch});
-			else
-				e->second.e = ve;
+			auto pend = [&](const VoxelEdit &ve) {
+				const uint64_t vkey = (uint64_t(ve.slot) << 13) | ve.local;
+				auto e = edits.find(vkey);
+				if (e == edits.end())
+					edits.emplace(vkey, PendingEdit{ve, r->epoch});
+				else
+					e->second.e = ve;
+			};
+			pend(VoxelEdit{c.slot, local, r->block, r->light1});
+			if (relight()) {
+				// the sunlight flood of the edited column (ChunkSetBlockTask.cpp:94, ChunkFloodSunlightTask.cpp:38-54,
+				// once the chunk above is loaded): below a chunk column that is not sunlit to its bottom nothing is
+				// sunlit; else sunlight reaches down to the lowest run of blocks that pass it vertically
+				c.blocks[local] = r->block;
+				auto up = world.find(chunk_key(cx, cy + 1, cz));
+				if (up != world.end()) {
+					const uint32_t xz = uint32_t(lx) + 16u * uint32_t(lz);
+					int h = 16;
+					if (up->second.heights[xz] <= 0) {
+						h = 15;
+						while (h >= 0 && (info.light_info_host[c.blocks[xz + 256u * uint32_t(h)]] & kLightInfoVertical))
+							--h;
+						++h;
+					}
+					if (h != c.heights[xz]) { // ChunkSetSunlightTask.cpp:46-72: the chunk and all 26 neighbours
+						c.heights[xz] = int8_t(h);
+						pend(VoxelEdit{c.slot, HCM_CHUNK_VOXELS + xz, uint32_t(uint8_t(int8_t(h))), 0});
+						for (int s = 0; s < 27; ++s) {
+							int d[3];
+							slot_offset(s, d);
+							mark_stale(cx + d[0], cy + d[1], cz + d[2], r->targets);
+						}
+					}
+				}
+			}
 			if (relight()) {
 				// the neighbours within light distance 14: ChunkShouldRemesh (ChunkMeshTask.inl:9-15), the
 				// L1 distance of the voxel to the neighbour's bound [-1, 17)^3 below 15
@@ -322,6 +357,8 @@ struct hcm_server {
 						for (int d = 0; d < (oz[1] ? 2 : 1); ++d)
 							mark_stale(cx + ox[a], cy + oy[b], cz + oz[d], r->targets);
 			}
+			std::sort(r->targets.begin(), r->targets.end()); // a chunk marked by two rules counts once
+			r->targets.erase(std::unique(r->targets.begin(), r->targets.end()), r->targets.end());
 		}
 		waiting.push_back(std::move(r));
 	}
@@ -426,7 +463,7 @@ struct hcm_server {
 		b.epoch = applied;
 		if (n_ed) {
 			HCM_CHECK(hipMemcpyAsync(b.d_edits, b.h_edits, n_ed * sizeof(VoxelEdit), hipMemcpyHostToDevice, s_comp));
-			HCM_CHECK(launch_edits(d_blocks, d_light, b.d_edits, n_ed, s_comp));
+			HCM_CHECK(launch_edits(d_blocks, d_light, d_heights, b.d_edits, n_ed, s_comp));
 		}
 
 		if (!b.jobs.empty()) {

@@ -239,10 +239,32 @@ class LitWorld(World):
     def __init__(self, levels):
         super().__init__()
         self.levels = levels
+        from test_registry import load_fixture
+
+        reg = load_fixture()
+        self.vertical = reg["rows"][reg["row_of"], 12] == 2  # Block::GetVerticalLightPass by Block value
 
     def edit(self, x, y, z, block, light):
+        """Sets the block, then floods sunlight down the edited column when the chunk above is loaded
+        (ChunkFloodSunlightTask.cpp:38-54). Returns whether the column's sunlight height changed."""
         c = (x // 16, y // 16, z // 16)
-        self.chunks[c][0][(x % 16) + 16 * ((z % 16) + 16 * (y % 16))] = block
+        blocks, heights = self.chunks[c]
+        blocks[(x % 16) + 16 * ((z % 16) + 16 * (y % 16))] = block
+        up = (c[0], c[1] + 1, c[2])
+        if up not in self.chunks:
+            return False
+        xz = (x % 16) + 16 * (z % 16)
+        if self.chunks[up][1][xz] > 0:
+            h = 16
+        else:
+            h = 15
+            while h >= 0 and self.vertical[blocks[xz + 256 * h]]:
+                h -= 1
+            h += 1
+        if h == heights[xz]:
+            return False
+        heights[xz] = h
+        return True
 
     def mesh_alone(self, oracle, pos):
         nb = [self.chunks[tuple(int(v) for v in np.add(pos, d))] for d in workloads.SLOT_OFFSETS]
@@ -285,20 +307,27 @@ def test_server_relight_matches_mesh_task_alone(ctx, oracle, max_batch, max_infl
             log.append(("load", pos, b, h))
         say("serve relight: 120 edits")
         palette = np.array([0, 0, 1, 3, 9, 0x18, 0x10, 0x0E], np.uint16)  # air, stone, dirt, glass, glowstone, leaves, water
+        flooded = set()
         for k, (x, y, z, _, _) in enumerate(_edits(rng, 120)):
             blk = int(palette[rng.integers(len(palette))])
             srv.edit(x, y, z, blk, 0, tag=len(log))
-            world.edit(x, y, z, blk, 0)
+            if world.edit(x, y, z, blk, 0):
+                flooded.add(len(log))
             log.append(("edit", x, y, z, blk, 0))
             if k % 40 == 39:
                 srv.drain()
+        assert flooded, "some edits move a sunlight height"
         srv.drain()
         done = srv.poll()
         check_completions(done, len(log), log)
         for c in done:
             if c["kind"] == hcm.REQ_EDIT:
                 _, x, y, z = log[c["tag"]][:4]
-                assert c["remeshed"] == len(light_chunks(world, x, y, z))
+                want = light_chunks(world, x, y, z)
+                if c["tag"] in flooded:  # a moved sunlight height remeshes the chunk and its 26 neighbours
+                    want |= {n for n in (tuple(int(v) for v in np.add((x // 16, y // 16, z // 16), d))
+                                         for d in workloads.SLOT_OFFSETS) if world.meshable(n)}
+                assert c["remeshed"] == len(want)
         n = check_history(srv, oracle, log, LitWorld(levels))
         say(f"serve relight: {n} recorded meshes equal the oracle's mesh task alone at their epochs")
         check_final(srv, oracle, world)
