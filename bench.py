@@ -126,6 +126,34 @@ def relight_leg(ctx, hcm, workloads, meshdigest, steps, cpu_threads, with_cpu):
     return res
 
 
+def draw_leg(hcm, workloads, batch, n_meshes):
+    """The draw list after meshing: mesh infos of the cave batch's meshes (WorldRenderer::PushChunkMesh) and
+    frustum culling into indirect draws (chunk_culling.comp) for 20 seeded cameras; wall time per call,
+    host copies of the lists included."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import draw_oracle
+
+    pos = workloads.region_job_positions()
+    batch.mesh_infos(pos)
+    t0 = time.perf_counter()
+    for _ in range(5):
+        infos = batch.mesh_infos(pos)
+    t_info = (time.perf_counter() - t0) / 5
+    rng = np.random.default_rng(7)
+    cams = []
+    for _ in range(20):
+        eye = rng.uniform(-40, 300, 3).astype(np.float32)
+        cams.append((eye, draw_oracle.look_at_projection(eye, rng.uniform(0, 256, 3))))
+    kept = 0
+    t0 = time.perf_counter()
+    for eye, vp in cams:
+        o, t = batch.cull(eye, vp)
+        kept += len(o) + len(t)
+    t_cull = (time.perf_counter() - t0) / len(cams)
+    return {"meshes": int(n_meshes), "infos": int(len(infos)), "mesh_infos_ms": round(t_info * 1e3, 4),
+            "cull_ms": round(t_cull * 1e3, 4), "kept_per_camera": round(kept / len(cams), 1)}
+
+
 def measured_traffic():
     """HBM bytes per persistent launch of the cave batch, from the committed rocprofv3 counter summary."""
     path = os.path.join(ROOT, "profiles", "mesh_kernel_cave_region.md")
@@ -345,6 +373,7 @@ def main():
                      "traffic_source": "profiles/mesh_kernel_cave_region.md (FETCH_SIZE + WRITE_SIZE, copy-scaled)",
                      "kernel_ms": round(k_ms, 4), "algorithmic_bytes": int(alg_bytes)},
     }
+    res["draw_lists"] = draw_leg(hcm, workloads, batch, int(meshes))
     batch.close()
     store.close()
     if not args.no_mixed:

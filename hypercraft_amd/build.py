@@ -46,7 +46,7 @@ def build(verbose=False):
     os.makedirs(OUT, exist_ok=True)
     src = lambda n: os.path.join(CSRC, n)
     obj = lambda n: os.path.join(OUT, n + ".o")
-    hip_units = ["hcm_kernels.hip", "hcm_light.hip", "hcm_api.cpp", "hcm_serve.cpp"]
+    hip_units = ["hcm_kernels.hip", "hcm_light.hip", "hcm_draw.hip", "hcm_api.cpp", "hcm_serve.cpp"]
     cxx_units = ["hcm_registry.cpp", "hcm_workload.cpp"]
     cmds = [[HIPCC, *HIP_FLAGS, "-c", src(u), "-o", obj(u)] for u in hip_units]
     cmds += [["g++", *CXX_FLAGS, "-c", src(u), "-o", obj(u)] for u in cxx_units]

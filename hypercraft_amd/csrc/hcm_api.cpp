@@ -166,6 +166,8 @@ int build_tables(const hcm_registry *r, DeviceTables *t) {
 }
 
 constexpr uint32_t kPlainPoolQuads = 1u << 17; // > the most quads one 16^3 chunk can produce
+static_assert(sizeof(hcm_mesh_info) == 52, "chunk_culling.comp MeshInfo layout (std430)");
+static_assert(sizeof(hcm_draw_cmd) == 20, "VkDrawIndexedIndirectCommand layout");
 constexpr uint32_t kPlainRecs = 64;
 
 } // namespace
@@ -242,6 +244,13 @@ struct hcm_batch {
 	Counters *d_lctr = nullptr;
 	uint8_t *d_halo_light = nullptr;  // [n_jobs][kLightJobBytes]
 	bool light_ran = false;
+	// draw lists
+	int32_t *d_chunk_pos = nullptr;
+	hcm_mesh_info *d_infos = nullptr;
+	hcm_draw_cmd *d_draws = nullptr; // [2][rec_cap]: opaque, transparent
+	uint32_t *d_draw_counts = nullptr;
+	uint32_t n_infos = 0;
+	bool infos_ready = false;
 };
 
 extern "C" int hcm_last_hip_error(void) { return g_last_hip; }
@@ -742,6 +751,10 @@ extern "C" void hcm_batch_destroy(hcm_batch *b) {
 	dfree(b->d_read_slots);
 	dfree(b->d_lctr);
 	dfree(b->d_halo_light);
+	dfree(b->d_chunk_pos);
+	dfree(b->d_infos);
+	dfree(b->d_draws);
+	dfree(b->d_draw_counts);
 	delete b;
 }
 
@@ -842,6 +855,7 @@ extern "C" int hcm_batch_run(hcm_batch *b, uint32_t mode) {
 		HCM_HIP(hipEventRecord(b->ev1, st));
 	}
 	b->ran = true;
+	b->infos_ready = false;
 	b->light_ran = b->light_mode == HCM_LIGHT_REBUILD;
 	b->last_mode = mode;
 	return HCM_OK;
@@ -887,6 +901,66 @@ extern "C" int hcm_batch_fetch(hcm_batch *b, uint32_t job, hcm_host_result *out)
 	HCM_HIP(hipMemcpyAsync(&res, b->d_res + job, sizeof(res), hipMemcpyDeviceToHost, st));
 	HCM_HIP(hipStreamSynchronize(st));
 	return fetch_job(st, res, b->d_recs, b->ctx->d_verts, b->ctx->d_idx, out);
+}
+
+extern "C" int hcm_batch_mesh_infos(hcm_batch *b, const int32_t *chunk_pos, hcm_mesh_info *out, uint32_t cap,
+                                    uint32_t *n) {
+	if (!b || !chunk_pos || !n || !b->ran)
+		return HCM_ERR_INVALID;
+	hcm_ctx *c = b->ctx;
+	std::lock_guard<std::mutex> lk(c->mu);
+	hipStream_t st = c->stream;
+	if (!b->d_infos) {
+		HCM_HIP(dmalloc(&b->d_chunk_pos, 3ull * b->n_jobs));
+		HCM_HIP(dmalloc(&b->d_infos, b->rec_cap));
+		HCM_HIP(dmalloc(&b->d_draws, 2ull * b->rec_cap));
+		HCM_HIP(dmalloc(&b->d_draw_counts, 2));
+	}
+	Counters ctr;
+	HCM_HIP(hipMemcpyAsync(&ctr, b->d_ctr, sizeof(ctr), hipMemcpyDeviceToHost, st));
+	HCM_HIP(hipMemcpyAsync(b->d_chunk_pos, chunk_pos, 12ull * b->n_jobs, hipMemcpyHostToDevice, st));
+	HCM_HIP(hipStreamSynchronize(st));
+	const uint32_t nr = std::min<uint32_t>(ctr.mesh_cursor, b->rec_cap);
+	HCM_HIP(launch_mesh_infos(b->d_recs, nr, b->d_chunk_pos, b->d_infos, st));
+	*n = nr;
+	b->n_infos = nr;
+	b->infos_ready = true;
+	if (out) {
+		if (nr > cap)
+			return HCM_ERR_CAPACITY;
+		HCM_HIP(hipMemcpyAsync(out, b->d_infos, (size_t)nr * sizeof(hcm_mesh_info), hipMemcpyDeviceToHost, st));
+	}
+	HCM_HIP(hipStreamSynchronize(st));
+	return HCM_OK;
+}
+
+extern "C" int hcm_batch_cull(hcm_batch *b, const float view_position[3], const float view_projection[16],
+                              hcm_draw_cmd *opaque, uint32_t opaque_cap, uint32_t *n_opaque, hcm_draw_cmd *transparent,
+                              uint32_t transparent_cap, uint32_t *n_transparent) {
+	if (!b || !view_position || !view_projection || !n_opaque || !n_transparent || !b->infos_ready)
+		return HCM_ERR_INVALID;
+	hcm_ctx *c = b->ctx;
+	std::lock_guard<std::mutex> lk(c->mu);
+	hipStream_t st = c->stream;
+	CullParams cp;
+	std::memcpy(cp.view_pos, view_position, sizeof(cp.view_pos));
+	std::memcpy(cp.view_proj, view_projection, sizeof(cp.view_proj));
+	HCM_HIP(hipMemsetAsync(b->d_draw_counts, 0, 2 * sizeof(uint32_t), st));
+	HCM_HIP(launch_cull(b->d_infos, b->n_infos, cp, b->d_draws, b->d_draws + b->rec_cap, b->d_draw_counts, st));
+	uint32_t counts[2];
+	HCM_HIP(hipMemcpyAsync(counts, b->d_draw_counts, sizeof(counts), hipMemcpyDeviceToHost, st));
+	HCM_HIP(hipStreamSynchronize(st));
+	*n_opaque = counts[0];
+	*n_transparent = counts[1];
+	if ((opaque && counts[0] > opaque_cap) || (transparent && counts[1] > transparent_cap))
+		return HCM_ERR_CAPACITY;
+	if (opaque && counts[0])
+		HCM_HIP(hipMemcpyAsync(opaque, b->d_draws, counts[0] * sizeof(hcm_draw_cmd), hipMemcpyDeviceToHost, st));
+	if (transparent && counts[1])
+		HCM_HIP(hipMemcpyAsync(transparent, b->d_draws + b->rec_cap, counts[1] * sizeof(hcm_draw_cmd),
+		                       hipMemcpyDeviceToHost, st));
+	HCM_HIP(hipStreamSynchronize(st));
+	return HCM_OK;
 }
 
 extern "C" int hcm_batch_download(hcm_batch *b, hcm_job_result *results, hcm_mesh_rec *recs, uint32_t rec_cap,

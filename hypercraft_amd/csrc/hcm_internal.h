@@ -111,6 +111,15 @@ hipError_t launch_mesh(const MeshParams &p, int threads, int grid, hipStream_t s
 hipError_t launch_edits(uint16_t *blocks, uint8_t *light, int8_t *heights, const VoxelEdit *edits, uint32_t n,
                         hipStream_t stream);
 int mesh_lds_bytes(int threads);
+// Launchers (hcm_draw.hip)
+struct CullParams {
+	float view_pos[3];
+	float view_proj[16]; // column-major
+};
+hipError_t launch_mesh_infos(const hcm_mesh_rec *recs, uint32_t n, const int32_t *chunk_pos, hcm_mesh_info *out,
+                             hipStream_t stream);
+hipError_t launch_cull(const hcm_mesh_info *infos, uint32_t n, const CullParams &cp, hcm_draw_cmd *opaque,
+                       hcm_draw_cmd *transparent, uint32_t *counts, hipStream_t stream);
 // Launchers (hcm_light.hip)
 hipError_t launch_light_prep(const LightParams &p, hipStream_t stream);
 hipError_t launch_light(const LightParams &p, int grid, hipStream_t stream);

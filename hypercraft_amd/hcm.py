@@ -100,6 +100,10 @@ REQ_LOAD = 1
 REQ_EDIT = 2
 
 JOB_RESULT_DTYPE = np.dtype([("status", "<i4"), ("first_mesh", "<u4"), ("mesh_count", "<u4"), ("quad_count", "<u4")])
+MESH_INFO_DTYPE = np.dtype([("index_count", "<u4"), ("first_index", "<u4"), ("vertex_offset", "<u4"),
+                            ("aabb", "<f4", (6,)), ("base_position", "<i4", (3,)), ("transparent", "<u4")])
+DRAW_CMD_DTYPE = np.dtype([("index_count", "<u4"), ("instance_count", "<u4"), ("first_index", "<u4"),
+                           ("vertex_offset", "<u4"), ("first_instance", "<u4")])
 MESH_REC_DTYPE = np.dtype([("aabb", "<u4", (6,)), ("first_quad", "<u4"), ("quad_count", "<u4"), ("job", "<u4"),
                            ("transparent", "<u4")])
 
@@ -127,6 +131,9 @@ SIGNATURES = [
     ("hcm_batch_totals", ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]),
     ("hcm_batch_download", ctypes.c_int, [_VP, _VP, _VP, ctypes.c_uint32, _VP, _VP, ctypes.c_uint64]),
     ("hcm_batch_set_light", ctypes.c_int, [_VP, ctypes.c_uint32]),
+    ("hcm_batch_mesh_infos", ctypes.c_int, [_VP, _VP, _VP, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]),
+    ("hcm_batch_cull", ctypes.c_int, [_VP, _VP, _VP, _VP, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32), _VP,
+                                      ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]),
     ("hcm_batch_fetch_light", ctypes.c_int, [_VP, ctypes.c_uint32, _VP]),
     ("hcm_set_tuning", ctypes.c_int, [_VP, ctypes.POINTER(TuningC)]),
     ("hcm_get_tuning", ctypes.c_int, [_VP, ctypes.POINTER(TuningC)]),
@@ -340,6 +347,27 @@ class Batch:
     def set_light(self, mode):
         """LIGHT_STORE: mesh with the store's light bytes; LIGHT_REBUILD: rebuild light first."""
         check("hcm_batch_set_light", lib().hcm_batch_set_light(self._ptr, mode))
+
+    def mesh_infos(self, chunk_pos):
+        """The mesh info of every mesh record of the last run (chunk_pos: (n_jobs, 3) chunk coordinates)."""
+        pos = np.ascontiguousarray(chunk_pos, np.int32).reshape(self.n_jobs, 3)
+        n = ctypes.c_uint32()
+        check("hcm_batch_mesh_infos", lib().hcm_batch_mesh_infos(self._ptr, pos.ctypes.data, None, 0, ctypes.byref(n)))
+        out = np.zeros(max(n.value, 1), MESH_INFO_DTYPE)
+        check("hcm_batch_mesh_infos", lib().hcm_batch_mesh_infos(self._ptr, pos.ctypes.data, out.ctypes.data,
+                                                                  out.shape[0], ctypes.byref(n)))
+        return out[:n.value]
+
+    def cull(self, view_position, view_projection, cap=1 << 20):
+        """Frustum culling of the last mesh_infos: (opaque, transparent) draw commands."""
+        vp = np.ascontiguousarray(view_position, np.float32).reshape(3)
+        m = np.ascontiguousarray(view_projection, np.float32).reshape(16)
+        o = np.zeros(cap, DRAW_CMD_DTYPE)
+        t = np.zeros(cap, DRAW_CMD_DTYPE)
+        no, nt = ctypes.c_uint32(), ctypes.c_uint32()
+        check("hcm_batch_cull", lib().hcm_batch_cull(self._ptr, vp.ctypes.data, m.ctypes.data, o.ctypes.data, cap,
+                                                      ctypes.byref(no), t.ctypes.data, cap, ctypes.byref(nt)))
+        return o[:no.value], t[:nt.value]
 
     def fetch_light(self, job):
         """One job's rebuilt light over [-1, 17)^3 after a LIGHT_REBUILD run: (18, 18, 18) u8 [y, z, x]."""

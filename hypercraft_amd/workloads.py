@@ -237,3 +237,8 @@ def lit_terrain_chunk(seed=LIGHT_SEED, glow=0.002):
 def slice_heights(heights, nbr, job):
     """The 27 sunlight-height arrays of one job."""
     return [heights[s] for s in nbr[job]]
+
+
+def region_job_positions(n=16):
+    """Chunk coordinates (J, 3) int32 of the jobs of cave_region / lit_cave_region (region chunks 0..n-1)."""
+    return (_grid_coords((1, 1, 1), (n, n, n)) - 1).astype(np.int32)

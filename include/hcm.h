@@ -215,6 +215,31 @@ int hcm_batch_set_light(hcm_batch *batch, uint32_t light_mode);
  * reads, index (x+1) + 18 ((z+1) + 18 (y+1)), light bytes sun | torch << 4 (block/include/block/Light.hpp). */
 int hcm_batch_fetch_light(hcm_batch *batch, uint32_t job, uint8_t *halo);
 
+/* Draw lists for a renderer. WorldRenderer::PushChunkMesh (client/include/client/WorldRenderer.hpp:69-87)
+ * gives every BlockMesh a ChunkMeshInfo (client/include/client/ChunkMesh.hpp:12-17), and the culling pass
+ * (client/shader/src/chunk_culling.comp) turns the visible ones into indexed indirect draws. */
+typedef struct hcm_mesh_info { /* chunk_culling.comp:8-13 (std430, 52 bytes) */
+	uint32_t index_count, first_index, vertex_offset; /* into the context's quad pool */
+	float aabb[6];             /* world space: AABB<uint32_t> / 16 + base_position (WorldRenderer.hpp:72-79) */
+	int32_t base_position[3];  /* chunk position * 16 */
+	uint32_t transparent;
+} hcm_mesh_info;
+typedef struct hcm_draw_cmd { /* VkDrawIndexedIndirectCommand (chunk_culling.comp:14-16) */
+	uint32_t index_count, instance_count, first_index, vertex_offset, first_instance; /* first_instance: info index */
+} hcm_draw_cmd;
+/* After a run: the mesh info of each of the run's mesh records (in record order; hcm_batch_download's
+ * records), for jobs at chunk positions chunk_pos[job][3]. Kept on the device for hcm_batch_cull; copied
+ * to out when out is not NULL (cap entries). *n receives the count. */
+int hcm_batch_mesh_infos(hcm_batch *batch, const int32_t *chunk_pos, hcm_mesh_info *out, uint32_t cap, uint32_t *n);
+/* Frustum culling of the infos from hcm_batch_mesh_infos (chunk_culling.comp:37-88, 92-132, without the
+ * depth-hierarchy test): a mesh is culled when all eight AABB corners lie outside one clip plane or behind
+ * the eye. view_projection is column-major (GLSL mat4). Writes the draws of the kept opaque and transparent
+ * meshes, in no particular order (the shader's atomics); *n_opaque / *n_transparent receive the counts
+ * (HCM_ERR_CAPACITY when a list does not fit its cap). */
+int hcm_batch_cull(hcm_batch *batch, const float view_position[3], const float view_projection[16],
+                   hcm_draw_cmd *opaque, uint32_t opaque_cap, uint32_t *n_opaque, hcm_draw_cmd *transparent,
+                   uint32_t transparent_cap, uint32_t *n_transparent);
+
 /* Launch shape of the meshing kernel. 0 = default. */
 typedef struct hcm_tuning {
 	uint32_t wg_per_cu; /* persistent workgroups per compute unit */
