@@ -78,8 +78,9 @@ struct MeshParams {
 
 // ---- light rebuild before meshing (client/src/ChunkMeshTask.cpp:48-111) ----
 // Per store slot, the inputs of the light rebuild packed as 16-bit x-rows (row z + 16 y, bit x):
-// plane 0 the indirect light pass, plane 1 sunlight, planes 2..5 the bits of the torchlight level.
-constexpr int kLightPlanes = 6;
+// plane 0 the indirect light pass, plane 1 sunlight, planes 2..5 the bits of the torchlight level,
+// plane 6 the cells of torchlight level 15.
+constexpr int kLightPlanes = 7;
 constexpr uint32_t kLightRowBytes = 20;                        // one halo x-row of rebuilt light (18 used)
 constexpr uint32_t kLightJobBytes = 18 * 18 * kLightRowBytes; // rows hz + 18 hy
 constexpr uint8_t kLightInfoPass = 0x10;     // light_info: level | kLightInfoPass when the block passes light

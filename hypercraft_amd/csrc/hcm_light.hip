@@ -17,8 +17,8 @@
 //
 // Two kernels:
 //   light_prep_kernel  one workgroup per store slot the batch reads: packs its blocks' light pass, its
-//                      sunlit cells (from the sunlight heights) and its torchlight levels into 16-bit
-//                      x-rows (3 KB per slot);
+//                      sunlit cells (from the sunlight heights), its torchlight levels and its level-15
+//                      torchlight sources into 16-bit x-rows (3.5 KB per slot);
 //   light_kernel       one workgroup per job (persistent, per-XCD work queues): assembles the 46^3 window
 //                      from the 27 slots' packed rows in LDS, runs the sunlight and torchlight levels,
 //                      and writes the 18^3 rebuilt light bytes (sun | torch << 4) the mesher reads.
@@ -41,6 +41,7 @@ constexpr int kLightThreads = 512;
 struct LightSmem {
 	uint64_t pass[kWRows];    // cells that pass light indirectly (Block::GetIndirectLightPass, Block.hpp:70)
 	uint64_t g[2][kWRows];    // G_{k+1} and G_k
+	uint64_t t15[kWRows];     // torchlight sources of level 15, kept for the torchlight channel
 	uint32_t lv[2][4][kHRows]; // per channel (sun, torch): bit planes of the level of each halo cell
 	uint32_t slots[27];
 	uint32_t levels; // torchlight levels present in the 27 chunks
@@ -139,17 +140,19 @@ template <int NT> __device__ __forceinline__ void light_job(const LightParams &p
 		if (tid == 0)
 			s.levels = lv & ~1u;
 	}
+	__syncthreads();
+	const bool torch = s.levels != 0, torch15 = (s.levels >> 15) & 1u;
 	for (int r = tid; r < kWRows; r += NT) {
 		const RowSrc src = row_src(p, s, r / kW, r % kW);
 		s.pass[r] = plane_row(src, 0);
 		s.g[0][r] = plane_row(src, 1); // sunlight: 15 at and above the sunlight height (Chunk.hpp:73-79)
+		s.t15[r] = torch15 ? plane_row(src, 6) : 0ull;
 	}
 	__syncthreads();
 	propagate<NT>(p, s, 0, false);
-	const bool torch = s.levels != 0;
 	if (torch) {
 		for (int r = tid; r < kWRows; r += NT)
-			s.g[0][r] = level_row(row_src(p, s, r / kW, r % kW), 15);
+			s.g[0][r] = s.t15[r];
 		__syncthreads();
 		propagate<NT>(p, s, 1, true);
 	}
@@ -193,7 +196,7 @@ __global__ __launch_bounds__(256) void light_prep_kernel(LightParams p) {
 #pragma unroll
 	for (int x = 0; x < 16; ++x)
 		info[x] = p.light_info[(bw[x >> 1] >> (16 * (x & 1))) & 0xffffu];
-	uint32_t pass = 0, sun = 0, t0 = 0, t1 = 0, t2 = 0, t3 = 0, lv = 0;
+	uint32_t pass = 0, sun = 0, t0 = 0, t1 = 0, t2 = 0, t3 = 0, t15 = 0, lv = 0;
 #pragma unroll
 	for (int x = 0; x < 16; ++x) {
 		const int h = int(int8_t(hw[x >> 2] >> (8 * (x & 3))));
@@ -201,6 +204,7 @@ __global__ __launch_bounds__(256) void light_prep_kernel(LightParams p) {
 		pass |= ((info[x] & kLightInfoPass) ? 1u : 0u) << x;
 		const uint32_t l = info[x] & 15u;
 		t0 |= (l & 1u) << x, t1 |= ((l >> 1) & 1u) << x, t2 |= ((l >> 2) & 1u) << x, t3 |= ((l >> 3) & 1u) << x;
+		t15 |= (l == 15u ? 1u : 0u) << x;
 		lv |= 1u << l;
 	}
 	uint16_t *dst = p.packed + (size_t)slot * kLightPlanes * 256 + row;
@@ -210,6 +214,7 @@ __global__ __launch_bounds__(256) void light_prep_kernel(LightParams p) {
 	dst[768] = uint16_t(t1);
 	dst[1024] = uint16_t(t2);
 	dst[1280] = uint16_t(t3);
+	dst[1536] = uint16_t(t15);
 	__syncthreads();
 	if (lv & ~1u)
 		atomicOr(&s_levels, lv & ~1u);
