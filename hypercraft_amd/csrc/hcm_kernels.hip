@@ -159,25 +159,37 @@ __device__ __forceinline__ uint32_t div_cnt(uint32_t x, uint32_t cnt) {
 	return (x * r) >> 16;
 }
 
-// One vertex of light4_init (:114-162) from its side, corner and side cells and the front cell.
+// AO of a light4_init vertex (:126-130) by side, corner and side cell direct-pass bits and the front cell's:
+// two bits per index d0 | d1 << 1 | d2 << 2 | pass3 << 3, 0 when both side cells or the corner and front
+// cells block, else 3 minus the number of blocking cells.
+constexpr uint32_t ao_table() {
+	uint32_t t = 0;
+	for (uint32_t i = 0; i < 16; ++i) {
+		const int d0 = i & 1, d1 = (i >> 1) & 1, d2 = (i >> 2) & 1, p = (i >> 3) & 1;
+		const int a = 3 - !d0 - !d1 - !d2 - !p;
+		t |= ((!d0 && !d2) || (!d1 && !p) || a < 0 ? 0u : uint32_t(a)) << (2 * i);
+	}
+	return t;
+}
+constexpr uint32_t kAoTable = ao_table();
+static_assert(kAoTable == 0xe9849040u, "AO table");
+
+// One vertex of light4_init (:114-162) from its side, corner and side cells and the front cell, without
+// branches: the sunlight and torchlight nibbles are summed as one packed word (sunlight in bits 0-7,
+// torchlight in bits 16-23), a side or corner cell counting only if it passes indirect light and one of
+// the side cells does (:132-161).
 __device__ __forceinline__ void light_vertex(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t pass3, uint32_t fs,
                                              uint32_t ft, uint32_t &ao, uint32_t &sun, uint32_t &torch) {
-	const uint32_t d0 = (c0 >> 8) & 1u, d1 = (c1 >> 8) & 1u, d2 = (c2 >> 8) & 1u;
-	const uint32_t i0 = (c0 >> 9) & 1u, i1 = (c1 >> 9) & 1u, i2 = (c2 >> 9) & 1u;
-	if ((!d0 && !d2) || (!d1 && !pass3))
-		ao = 0;
-	else {
-		const int a = 3 - int(!d0) - int(!d1) - int(!d2) - int(!pass3);
-		ao = a < 0 ? 0u : uint32_t(a);
-	}
-	uint32_t cnt = 1, ss = fs, tt = ft;
-	if (i0 | i2) {
-		cnt += i0 + i1 + i2;
-		ss += (i0 ? (c0 & 15u) : 0u) + (i1 ? (c1 & 15u) : 0u) + (i2 ? (c2 & 15u) : 0u);
-		tt += (i0 ? ((c0 >> 4) & 15u) : 0u) + (i1 ? ((c1 >> 4) & 15u) : 0u) + (i2 ? ((c2 >> 4) & 15u) : 0u);
-	}
-	sun = div_cnt(ss << 2, cnt);
-	torch = div_cnt(tt << 2, cnt);
+	const uint32_t idx = ((c0 >> 8) & 1u) | ((c1 >> 7) & 2u) | ((c2 >> 6) & 4u) | (pass3 << 3);
+	ao = (kAoTable >> (2 * idx)) & 3u;
+	const uint32_t g = ((c0 | c2) >> 9) & 1u;
+	const uint32_t m0 = 0u - (((c0 >> 9) & 1u) & g), m1 = 0u - (((c1 >> 9) & 1u) & g),
+	               m2 = 0u - (((c2 >> 9) & 1u) & g);
+	const uint32_t sum = (fs | (ft << 16)) + (((c0 & 15u) | ((c0 & 0xf0u) << 12)) & m0) +
+	                     (((c1 & 15u) | ((c1 & 0xf0u) << 12)) & m1) + (((c2 & 15u) | ((c2 & 0xf0u) << 12)) & m2);
+	const uint32_t cnt = 1u + (m0 & 1u) + (m1 & 1u) + (m2 & 1u);
+	sun = div_cnt((sum & 0xffu) << 2, cnt);
+	torch = div_cnt((sum >> 16) << 2, cnt);
 }
 
 // light4_init (:48-163) for face `face` of the block at halo index self. Returns the packed Light4:
@@ -194,9 +206,8 @@ __device__ __forceinline__ uint2 light4(const uint32_t *vox, int face, int self)
 	const bool wa = face == 0 || face == 3 || face == 4;
 	const uint32_t pass3 = (F >> 8) & 1u, fs = F & 15u, ft = (F >> 4) & 15u;
 	uint32_t lo = 0, hi = 0, ao4 = 0;
-	// one vertex at a time (small register footprint): (P, Q) signs (-,-), then (+,-) or (-,+) by the
-	// winding, then (+,+), then the other of (+,-) and (-,+)
-#pragma unroll 1
+	// (P, Q) signs (-,-), then (+,-) or (-,+) by the winding, then (+,+), then the other of (+,-) and (-,+)
+#pragma unroll
 	for (int k = 0; k < 4; ++k) {
 		const bool pplus = k == 2 || (k == 1 && wa) || (k == 3 && !wa);
 		const bool qplus = k == 2 || (k == 1 && !wa) || (k == 3 && wa);
