@@ -133,6 +133,13 @@ template <int NT> struct Smem {
 };
 
 constexpr uint32_t kFlagNonUniform = 1u, kFlagCustom = 2u, kFlagFlow = 4u;
+// kFlagMixed: some halo voxel is not an opaque cube (opaque class on all six faces, no custom or flow
+// mesh); kFlagInnerFaces: some chunk voxel has a non-empty face class or a custom or flow mesh. A chunk
+// without either has no visible face: BlockTexture::Show never shows an empty face, nor an opaque face
+// against an opaque one (texture/include/texture/BlockTexture.hpp:68-76).
+constexpr uint32_t kFlagMixed = 8u, kFlagInnerFaces = 16u;
+constexpr uint32_t kVoxShape = 0xfffc00u;                      // the face classes and the mesh and flow bits
+constexpr uint32_t kVoxOpaqueCube = 0x555u << kVoxClassShift; // opaque on all six faces
 
 // ------------------------------------------------------------------------------------------------
 // small helpers
@@ -1003,11 +1010,14 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 		for (int x = 0; x < 18; ++x)
 			pv[x] = s.pid_vox[pids[x]];
 		const bool inside_yz = (unsigned)y < 16u && (unsigned)z < 16u;
-		uint32_t mesh_bits = 0;
+		uint32_t mesh_bits = 0, mixed = 0, inner = 0;
 		uint32_t *dst = s.vox + row * kHalo;
 #pragma unroll
 		for (int x = 0; x < 18; ++x) {
 			dst[x] = lt[x] | pv[x];
+			mixed |= (pv[x] ^ kVoxOpaqueCube) & kVoxShape;
+			if (x >= 1 && x <= 16)
+				inner |= pv[x] & kVoxShape;
 			if (blk[x] != wave_ref)
 				my_flags |= kFlagNonUniform;
 			if (pv[x] & kVoxMesh) {
@@ -1022,6 +1032,10 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 			atomicOr(&s.custom_bits[cell0 >> 5], mesh_bits << (cell0 & 31));
 			my_flags |= kFlagCustom;
 		}
+		if (mixed)
+			my_flags |= kFlagMixed;
+		if (inside_yz && inner)
+			my_flags |= kFlagInnerFaces;
 	}
 	if (my_flags)
 		atomicOr(&s.flags, my_flags);
@@ -1034,8 +1048,10 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 		if (s.wave_ref[w] != 0xffffffffu && s.wave_ref[w] != s.wave_ref[0])
 			flags |= kFlagNonUniform;
 
-	if (!(flags & kFlagNonUniform) && !(flags & kFlagCustom)) {
-		// one block value everywhere: every face is hidden by an equal neighbour (BlockTexture::Show)
+	if (!(flags & kFlagCustom) &&
+	    (!(flags & kFlagNonUniform) || !(flags & kFlagMixed) || !(flags & kFlagInnerFaces))) {
+		// one block value everywhere (every face is hidden by an equal neighbour), opaque cubes only, or
+		// no chunk voxel with a face: no quads
 		if (tid == 0)
 			p.results[job] = hcm_job_result{HCM_OK, 0u, 0u, 0u};
 		return;
@@ -1092,6 +1108,11 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 			s.tpre[kTasks] = tot_a + ib;
 	}
 	__syncthreads();
+	if (s.tpre[kTasks] == 0 && !(flags & kFlagCustom)) { // no visible face and no custom mesh: no quads
+		if (tid == 0)
+			p.results[job] = hcm_job_result{HCM_OK, 0u, 0u, 0u};
+		return;
+	}
 #pragma unroll 1
 	for (int t0 = 0; t0 < kTasks;) {
 		if (tid == 0) { // the longest run of tasks whose faces fit (one task, at most 256 faces, always fits)
