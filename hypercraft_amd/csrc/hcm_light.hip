@@ -44,7 +44,8 @@ struct LightSmem {
 	uint64_t t15[kWRows];     // torchlight sources of level 15, kept for the torchlight channel
 	uint32_t lv[2][4][kHRows]; // per channel (sun, torch): bit planes of the level of each halo cell
 	uint32_t slots[27];
-	uint32_t levels; // torchlight levels present in the 27 chunks
+	uint32_t changed[3]; // per level step (rotating): some row of the step changed
+	uint32_t levels;     // torchlight levels present in the 27 chunks
 	uint32_t job;
 };
 
@@ -81,10 +82,14 @@ __device__ __forceinline__ uint64_t level_row(const RowSrc &r, uint32_t k) {
 	return m;
 }
 
-// One channel: G_15 is in s.g[0]; records each halo cell's level in s.lv[ch].
+// One channel: G_15 is in s.g[0]; records each halo cell's level in s.lv[ch]. A step that adds no cell
+// to its rows leaves G unchanged for every later step (each later step reads only rows of the earlier
+// step), so the channel ends there unless sources enter at a lower level.
 template <int NT>
 __device__ __forceinline__ void propagate(const LightParams &p, LightSmem &s, int ch, bool inject) {
 	const int tid = threadIdx.x;
+	if (tid < 3)
+		s.changed[tid] = 0;
 	for (int hr = tid; hr < kHRows; hr += NT) {
 		const int hy = hr / kH, hz = hr % kH;
 		const uint32_t bits = uint32_t(s.g[0][(hz + kHOff) + kW * (hy + kHOff)] >> kHOff) & 0x3ffffu;
@@ -101,6 +106,9 @@ __device__ __forceinline__ void propagate(const LightParams &p, LightSmem &s, in
 		const bool inj = inject && ((s.levels >> k) & 1u);
 		const uint64_t *gi = s.g[cur];
 		uint64_t *go = s.g[cur ^ 1];
+		if (tid == 0)
+			s.changed[(k + 2) % 3] = 0; // the flag of step k - 1, last read before the previous barrier
+		bool changed = false;
 		for (int i = tid; i < n * n; i += NT) {
 			const int qy = int((uint32_t(i) * inv) >> 20), wy = lo + qy, wz = lo + (i - qy * n);
 			const int r = wz + kW * wy;
@@ -110,6 +118,7 @@ __device__ __forceinline__ void propagate(const LightParams &p, LightSmem &s, in
 			if (inj)
 				nw |= level_row(row_src(p, s, wy, wz), k);
 			go[r] = nw;
+			changed |= nw != g;
 			const int hy = wy - kHOff, hz = wz - kHOff;
 			if ((unsigned)hy < (unsigned)kH && (unsigned)hz < (unsigned)kH) {
 				const uint32_t add = uint32_t((nw & ~g) >> kHOff) & 0x3ffffu;
@@ -122,8 +131,12 @@ __device__ __forceinline__ void propagate(const LightParams &p, LightSmem &s, in
 				}
 			}
 		}
+		if (__ballot(changed) && (tid & 63) == 0)
+			s.changed[k % 3] = 1;
 		__syncthreads();
 		cur ^= 1;
+		if (!s.changed[k % 3] && !(inject && (s.levels & ((1u << k) - 1u))))
+			break;
 	}
 }
 
