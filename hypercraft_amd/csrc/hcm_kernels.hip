@@ -1136,10 +1136,10 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 			reinterpret_cast<uint32_t *>(s.tp[t])[q] = 0;
 		}
 		// a lane keeps the positions of its faces (at most kListCap / NT of them) for the links, in a
-		// register queue shifted once per step (rolled loops, static register indices)
+		// register queue shifted once per step (loops unrolled by two, static register indices)
 		constexpr int kMine = (kListCap + NT - 1) / NT;
 		uint32_t mine[kMine] = {};
-#pragma unroll 1
+#pragma unroll 2
 		for (int k = 0; k < kMine; ++k) {
 			const uint32_t i = tid + k * NT;
 			uint32_t d = 0;
@@ -1151,7 +1151,7 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 			mine[kMine - 1] = d;
 		}
 		__syncthreads();
-#pragma unroll 1
+#pragma unroll 2
 		for (int k = 0; k < kMine; ++k) {
 			const uint32_t i = tid + k * NT;
 			if (i < n)
