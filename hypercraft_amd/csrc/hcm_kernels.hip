@@ -5,8 +5,9 @@
 //   1. staging: one thread per 18-voxel x-row of the 18^3 halo loads the row from its chunk slots with
 //      vector loads and writes one LDS word per voxel: light byte, direct/indirect light-pass bits,
 //      a 2-bit face class per face (empty / opaque / transparent / liquid), custom-mesh and flow bits
-//      and the texture row. A chunk whose halo holds a single block value and no custom mesh has no
-//      faces and stops here.
+//      and the texture row. A chunk without a custom mesh whose halo holds a single block value or only
+//      opaque cubes, or none of whose own voxels has a face, has no quads and stops here; so does one
+//      with no visible face after step 3's visibility masks.
 //   2. liquid-flow cells take their per-cell face textures (the reference's m_dynamic_block_textures).
 //   3. faces, per (axis, slice, direction) task, each a 16x16 mask:
 //      - visibility, bit-sliced: one lane per column of an axis keeps its cells' face classes as bit

@@ -134,3 +134,44 @@ def light27_from_window(win):
 def halo_of_window(win):
     """The (18, 18, 18) [y, z, x] cells over [-1, 17) of a (46, 46, 46) window over [-15, 31)."""
     return np.ascontiguousarray(win[14:32, 14:32, 14:32])
+
+
+# ---- chunks without faces, and the single changes that give them faces ----
+OPAQUE_CUBES = np.array([1, 2, 3, 4, 6, 8, 13, 18], np.uint16)  # stone .. plank: opaque on all six faces
+
+
+def faceless(seed=11):
+    """Jobs (each with its own 27 chunks) built to stop the mesher early or just miss it, with the quad
+    count each must give: 0 for opaque cubes only, for an air chunk inside any terrain, and for an air
+    chunk inside stone; > 0 once one glass, water or plant voxel enters the chunk or its rim."""
+    rng = np.random.default_rng(seed)
+    rim_pal = np.concatenate([FAMILIES["glassy"], FAMILIES["custom"], FAMILIES["flow"]])
+
+    def opaque():
+        return [OPAQUE_CUBES[rng.integers(0, OPAQUE_CUBES.size, 4096)] for _ in range(27)]
+
+    def air_in(pal):
+        b = [pal[rng.integers(0, pal.size, 4096)].astype(np.uint16) for _ in range(27)]
+        b[26] = np.zeros(4096, np.uint16)
+        return b
+
+    def put(b, slot, x, y, z, v):
+        b[slot] = b[slot].copy()
+        b[slot][x + 16 * (z + 16 * y)] = v
+        return b
+
+    jobs = [
+        ("opaque_mix", opaque(), 0),
+        ("air_in_terrain", air_in(rim_pal), 0),
+        ("air_in_stone", air_in(np.array([1], np.uint16)), 0),
+        ("opaque_mix_glass_inside", put(opaque(), 26, 7, 8, 9, 9), 1),
+        # slot 8 = 9 L(1) + 3 L(0) + L(0) is the +x neighbour (CmpXYZ2NeighbourIndex); its x = 0 is x = 16
+        ("opaque_mix_glass_rim", put(opaque(), 0 * 9 + 2 * 3 + 2, 0, 5, 6, 9), 1),
+        ("opaque_mix_plant", put(opaque(), 26, 3, 4, 5, 5), 1),
+        ("opaque_mix_water_rim", put(opaque(), 0 * 9 + 2 * 3 + 2, 0, 11, 2, 14), 1),
+        ("air_in_terrain_water", put(air_in(rim_pal), 26, 15, 0, 15, 14), 1),
+    ]
+    blocks = np.concatenate([np.stack(b) for _, b, _ in jobs])
+    light = rng.integers(0, 256, blocks.shape).astype(np.uint8)
+    nbr = np.arange(27 * len(jobs), dtype=np.uint32).reshape(len(jobs), 27)
+    return [n for n, _, _ in jobs], blocks, light, nbr, np.array([w for _, _, w in jobs])

@@ -112,6 +112,22 @@ def test_batched_persistent(ctx, golden, name):
     assert_digests(name, digests_of(out), golden[name + "_digest"])
 
 
+def test_faceless_chunks(ctx, oracle):
+    """The early exits (opaque cubes only, no chunk voxel with a face, no visible face) and the single
+    voxels that defeat them: batched and plain results equal the oracle's."""
+    names, blocks, light, nbr, want_faces = cases.faceless()
+    say(f"faceless chunks: {len(names)} jobs")
+    out = run_batch(ctx, blocks, light, nbr)
+    assert (out[0]["status"] == 0).all()
+    got = digests_of(out)
+    for j, name in enumerate(names):
+        b27, l27 = workloads.slice_jobs(blocks, light, nbr, j)
+        want = meshdigest.digest(oracle.mesh(b27, l27))
+        assert got[j].tobytes() == want, f"batched: {name}"
+        assert meshdigest.digest(ctx.mesh_chunk(b27, l27)) == want, f"plain: {name}"
+        assert (out[0]["quad_count"][j] > 0) == bool(want_faces[j]), name
+
+
 def test_mixed_uneven_batches(ctx, golden):
     """Uneven batches drawn from one store holding the cave region and the plant terrain: each batch
     is a captured graph run once and replayed, and per-chunk launches give the same meshes."""

@@ -73,6 +73,25 @@ def test_oracle_many_matches_single(oracle, golden):
     assert np.array_equal(q, golden["palette_custom_quads"].astype(np.uint64))
 
 
+def test_faceless_cases(oracle, request):
+    """Chunks of opaque cubes only, and air chunks inside any terrain, have no quads; one glass, water or
+    plant voxel in the chunk or on its rim gives quads (the cases of the mesher's early exits). Checked
+    against the compiled reference where it was built."""
+    names, blocks, light, nbr, want_faces = cases.faceless()
+    ref = None
+    try:
+        ref = request.getfixturevalue("reference")
+    except pytest.skip.Exception:
+        pass
+    for j, name in enumerate(names):
+        b27, l27 = workloads.slice_jobs(blocks, light, nbr, j)
+        got = oracle.mesh(b27, l27)
+        quads = sum(m.indices.size // 6 for m in got)
+        assert (quads > 0) == bool(want_faces[j]), f"{name}: {quads} quads"
+        if ref is not None:
+            assert meshdigest.digest(got) == meshdigest.digest(ref.mesh(b27, l27)), name
+
+
 # ---- closed forms ----
 def word0(x, y, z, axis):
     return (x * 16) | (y * 16) << 10 | (z * 16) << 20 | axis << 30
