@@ -209,6 +209,10 @@ struct hcm_ctx {
 	uint32_t *d_plevels = nullptr;
 	Counters *d_plctr = nullptr;
 	uint8_t *d_phalo = nullptr;
+	// depth hierarchy for occlusion culling (hcm_ctx_set_depth)
+	float *d_depth = nullptr;
+	size_t depth_cap = 0; // floats
+	DepthPyramid hiz;
 	hcm_tuning tuning{0, 256, {0, 0}};
 };
 
@@ -302,6 +306,7 @@ static void free_ctx(hcm_ctx *c) {
 	dfree(c->d_plevels);
 	dfree(c->d_plctr);
 	dfree(c->d_phalo);
+	dfree(c->d_depth);
 	if (c->h_pheights)
 		(void)hipHostFree(c->h_pheights);
 	if (c->stream)
@@ -934,17 +939,22 @@ extern "C" int hcm_batch_mesh_infos(hcm_batch *b, const int32_t *chunk_pos, hcm_
 	return HCM_OK;
 }
 
-extern "C" int hcm_batch_cull(hcm_batch *b, const float view_position[3], const float view_projection[16],
-                              hcm_draw_cmd *opaque, uint32_t opaque_cap, uint32_t *n_opaque, hcm_draw_cmd *transparent,
-                              uint32_t transparent_cap, uint32_t *n_transparent) {
+// Culling of the batch's mesh infos with or without the depth-hierarchy test.
+static int cull_batch(hcm_batch *b, const float view_position[3], const float view_projection[16], bool occlusion,
+                      hcm_draw_cmd *opaque, uint32_t opaque_cap, uint32_t *n_opaque, hcm_draw_cmd *transparent,
+                      uint32_t transparent_cap, uint32_t *n_transparent) {
 	if (!b || !view_position || !view_projection || !n_opaque || !n_transparent || !b->infos_ready)
 		return HCM_ERR_INVALID;
 	hcm_ctx *c = b->ctx;
 	std::lock_guard<std::mutex> lk(c->mu);
+	if (occlusion && !c->hiz.data)
+		return HCM_ERR_INVALID;
 	hipStream_t st = c->stream;
 	CullParams cp;
 	std::memcpy(cp.view_pos, view_position, sizeof(cp.view_pos));
 	std::memcpy(cp.view_proj, view_projection, sizeof(cp.view_proj));
+	if (occlusion)
+		cp.hiz = c->hiz;
 	HCM_HIP(hipMemsetAsync(b->d_draw_counts, 0, 2 * sizeof(uint32_t), st));
 	HCM_HIP(launch_cull(b->d_infos, b->n_infos, cp, b->d_draws, b->d_draws + b->rec_cap, b->d_draw_counts, st));
 	uint32_t counts[2];
@@ -960,6 +970,80 @@ extern "C" int hcm_batch_cull(hcm_batch *b, const float view_position[3], const 
 		HCM_HIP(hipMemcpyAsync(transparent, b->d_draws + b->rec_cap, counts[1] * sizeof(hcm_draw_cmd),
 		                       hipMemcpyDeviceToHost, st));
 	HCM_HIP(hipStreamSynchronize(st));
+	return HCM_OK;
+}
+
+extern "C" int hcm_batch_cull(hcm_batch *b, const float view_position[3], const float view_projection[16],
+                              hcm_draw_cmd *opaque, uint32_t opaque_cap, uint32_t *n_opaque, hcm_draw_cmd *transparent,
+                              uint32_t transparent_cap, uint32_t *n_transparent) {
+	return cull_batch(b, view_position, view_projection, false, opaque, opaque_cap, n_opaque, transparent,
+	                  transparent_cap, n_transparent);
+}
+
+extern "C" int hcm_batch_cull_occlusion(hcm_batch *b, const float view_position[3], const float view_projection[16],
+                                        hcm_draw_cmd *opaque, uint32_t opaque_cap, uint32_t *n_opaque,
+                                        hcm_draw_cmd *transparent, uint32_t transparent_cap, uint32_t *n_transparent) {
+	return cull_batch(b, view_position, view_projection, true, opaque, opaque_cap, n_opaque, transparent,
+	                  transparent_cap, n_transparent);
+}
+
+// Level count of the depth image's chain: myvk::ImageBase::QueryMipLevel (the bit length of width | height).
+static uint32_t depth_levels(uint32_t w, uint32_t h) {
+	uint32_t n = 0;
+	for (uint32_t x = w | h; x; x >>= 1)
+		++n;
+	return n;
+}
+
+extern "C" int hcm_ctx_set_depth(hcm_ctx *ctx, const float *depth, uint32_t width, uint32_t height) {
+	if (!ctx || !depth || width == 0 || height == 0 || width > 16384 || height > 16384 || (width | height) < 2)
+		return HCM_ERR_INVALID;
+	std::lock_guard<std::mutex> lk(ctx->mu);
+	DepthPyramid h;
+	h.n_levels = depth_levels(width, height);
+	size_t total = 0;
+	for (uint32_t i = 0, w = width, hh = height; i < h.n_levels; ++i) {
+		h.width[i] = w, h.height[i] = hh, h.offset[i] = (uint32_t)total;
+		total += (size_t)w * hh;
+		w = std::max(w >> 1, 1u), hh = std::max(hh >> 1, 1u);
+	}
+	hipStream_t st = ctx->stream;
+	if (total > ctx->depth_cap) {
+		HCM_HIP(hipStreamSynchronize(st));
+		dfree(ctx->d_depth);
+		ctx->d_depth = nullptr;
+		ctx->depth_cap = 0;
+		ctx->hiz = DepthPyramid{};
+		HCM_HIP(hipMalloc(&ctx->d_depth, total * sizeof(float)));
+		ctx->depth_cap = total;
+	}
+	HCM_HIP(hipMemcpyAsync(ctx->d_depth, depth, (size_t)width * height * sizeof(float), hipMemcpyHostToDevice, st));
+	for (uint32_t i = 1; i < h.n_levels; ++i)
+		HCM_HIP(launch_depth_level(ctx->d_depth + h.offset[i - 1], h.width[i - 1], h.height[i - 1],
+		                           ctx->d_depth + h.offset[i], h.width[i], h.height[i], st));
+	HCM_HIP(hipStreamSynchronize(st));
+	h.data = ctx->d_depth;
+	ctx->hiz = h;
+	return HCM_OK;
+}
+
+extern "C" int hcm_ctx_depth_level(hcm_ctx *ctx, uint32_t level, float *out, uint64_t cap, uint32_t *width,
+                                   uint32_t *height) {
+	if (!ctx || !width || !height)
+		return HCM_ERR_INVALID;
+	std::lock_guard<std::mutex> lk(ctx->mu);
+	if (!ctx->hiz.data || level >= ctx->hiz.n_levels)
+		return HCM_ERR_INVALID;
+	*width = ctx->hiz.width[level];
+	*height = ctx->hiz.height[level];
+	const uint64_t n = (uint64_t)*width * *height;
+	if (!out)
+		return HCM_OK;
+	if (n > cap)
+		return HCM_ERR_CAPACITY;
+	HCM_HIP(hipMemcpyAsync(out, ctx->hiz.data + ctx->hiz.offset[level], n * sizeof(float), hipMemcpyDeviceToHost,
+	                       ctx->stream));
+	HCM_HIP(hipStreamSynchronize(ctx->stream));
 	return HCM_OK;
 }
 

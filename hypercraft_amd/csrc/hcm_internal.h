@@ -113,10 +113,22 @@ hipError_t launch_edits(uint16_t *blocks, uint8_t *light, int8_t *heights, const
                         hipStream_t stream);
 int mesh_lds_bytes(int threads);
 // Launchers (hcm_draw.hip)
+// Depth hierarchy of the renderer's depth image (client/include/client/rg/DepthHierarchyPass.hpp:166-196):
+// level 0 is the image, level i >= 1 is max(level i - 1 size >> 1, 1) per axis; the culling shader's
+// texture holds levels 1 .. n_levels - 1 as its mips 0 .. n_levels - 2.
+constexpr int kMaxDepthLevels = 16; // images up to 16384 x 16384 have at most 15 levels
+struct DepthPyramid {
+	const float *data = nullptr; // every level, level i at data + offset[i], row y at + y * width[i]
+	uint32_t n_levels = 0;
+	uint32_t width[kMaxDepthLevels] = {}, height[kMaxDepthLevels] = {}, offset[kMaxDepthLevels] = {};
+};
 struct CullParams {
 	float view_pos[3];
 	float view_proj[16]; // column-major
+	DepthPyramid hiz;    // data == nullptr: frustum test only
 };
+hipError_t launch_depth_level(const float *prev, uint32_t pw, uint32_t ph, float *cur, uint32_t cw, uint32_t ch,
+                              hipStream_t stream);
 hipError_t launch_mesh_infos(const hcm_mesh_rec *recs, uint32_t n, const int32_t *chunk_pos, hcm_mesh_info *out,
                              hipStream_t stream);
 hipError_t launch_cull(const hcm_mesh_info *infos, uint32_t n, const CullParams &cp, hcm_draw_cmd *opaque,

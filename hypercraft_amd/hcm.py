@@ -134,6 +134,11 @@ SIGNATURES = [
     ("hcm_batch_mesh_infos", ctypes.c_int, [_VP, _VP, _VP, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]),
     ("hcm_batch_cull", ctypes.c_int, [_VP, _VP, _VP, _VP, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32), _VP,
                                       ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]),
+    ("hcm_batch_cull_occlusion", ctypes.c_int, [_VP, _VP, _VP, _VP, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32),
+                                                _VP, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]),
+    ("hcm_ctx_set_depth", ctypes.c_int, [_VP, _VP, ctypes.c_uint32, ctypes.c_uint32]),
+    ("hcm_ctx_depth_level", ctypes.c_int, [_VP, ctypes.c_uint32, _VP, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint32),
+                                           ctypes.POINTER(ctypes.c_uint32)]),
     ("hcm_batch_fetch_light", ctypes.c_int, [_VP, ctypes.c_uint32, _VP]),
     ("hcm_set_tuning", ctypes.c_int, [_VP, ctypes.POINTER(TuningC)]),
     ("hcm_get_tuning", ctypes.c_int, [_VP, ctypes.POINTER(TuningC)]),
@@ -253,6 +258,21 @@ class Context:
     def ptr(self):
         return self._ptr
 
+    def set_depth(self, depth):
+        """The depth hierarchy for occlusion culling, from a (height, width) float32 depth image."""
+        d = np.ascontiguousarray(depth, np.float32)
+        check("hcm_ctx_set_depth", lib().hcm_ctx_set_depth(self._ptr, d.ctypes.data, d.shape[1], d.shape[0]))
+
+    def depth_level(self, level):
+        """Level `level` of the depth hierarchy as a (height, width) float32 array."""
+        w, h = ctypes.c_uint32(), ctypes.c_uint32()
+        check("hcm_ctx_depth_level", lib().hcm_ctx_depth_level(self._ptr, level, None, 0, ctypes.byref(w),
+                                                               ctypes.byref(h)))
+        out = np.zeros((h.value, w.value), np.float32)
+        check("hcm_ctx_depth_level", lib().hcm_ctx_depth_level(self._ptr, level, out.ctypes.data, out.size,
+                                                               ctypes.byref(w), ctypes.byref(h)))
+        return out
+
     def close(self):
         if self._ptr:
             lib().hcm_destroy(self._ptr)
@@ -358,15 +378,17 @@ class Batch:
                                                                   out.shape[0], ctypes.byref(n)))
         return out[:n.value]
 
-    def cull(self, view_position, view_projection, cap=1 << 20):
-        """Frustum culling of the last mesh_infos: (opaque, transparent) draw commands."""
+    def cull(self, view_position, view_projection, cap=1 << 20, occlusion=False):
+        """Frustum culling of the last mesh_infos, and with occlusion=True the depth-hierarchy test against
+        the context's depth (Context.set_depth): (opaque, transparent) draw commands."""
         vp = np.ascontiguousarray(view_position, np.float32).reshape(3)
         m = np.ascontiguousarray(view_projection, np.float32).reshape(16)
         o = np.zeros(cap, DRAW_CMD_DTYPE)
         t = np.zeros(cap, DRAW_CMD_DTYPE)
         no, nt = ctypes.c_uint32(), ctypes.c_uint32()
-        check("hcm_batch_cull", lib().hcm_batch_cull(self._ptr, vp.ctypes.data, m.ctypes.data, o.ctypes.data, cap,
-                                                      ctypes.byref(no), t.ctypes.data, cap, ctypes.byref(nt)))
+        name = "hcm_batch_cull_occlusion" if occlusion else "hcm_batch_cull"
+        check(name, getattr(lib(), name)(self._ptr, vp.ctypes.data, m.ctypes.data, o.ctypes.data, cap,
+                                         ctypes.byref(no), t.ctypes.data, cap, ctypes.byref(nt)))
         return o[:no.value], t[:nt.value]
 
     def fetch_light(self, job):

@@ -232,13 +232,33 @@ typedef struct hcm_draw_cmd { /* VkDrawIndexedIndirectCommand (chunk_culling.com
  * to out when out is not NULL (cap entries). *n receives the count. */
 int hcm_batch_mesh_infos(hcm_batch *batch, const int32_t *chunk_pos, hcm_mesh_info *out, uint32_t cap, uint32_t *n);
 /* Frustum culling of the infos from hcm_batch_mesh_infos (chunk_culling.comp:37-88, 92-132, without the
- * depth-hierarchy test): a mesh is culled when all eight AABB corners lie outside one clip plane or behind
+ * depth-hierarchy test, which hcm_batch_cull_occlusion adds): a mesh is culled when all eight AABB corners lie outside one clip plane or behind
  * the eye. view_projection is column-major (GLSL mat4). Writes the draws of the kept opaque and transparent
  * meshes, in no particular order (the shader's atomics); *n_opaque / *n_transparent receive the counts
  * (HCM_ERR_CAPACITY when a list does not fit its cap). */
 int hcm_batch_cull(hcm_batch *batch, const float view_position[3], const float view_projection[16],
                    hcm_draw_cmd *opaque, uint32_t opaque_cap, uint32_t *n_opaque, hcm_draw_cmd *transparent,
                    uint32_t transparent_cap, uint32_t *n_transparent);
+/* The depth hierarchy the occlusion test samples (client/include/client/rg/DepthHierarchyPass.hpp:166-196,
+ * client/shader/src/depth_hierarchy.comp), built on the device from the renderer's depth image: width x
+ * height floats, texel (x, y) at depth[y * width + x], 2 <= (width | height), width and height at most
+ * 16,384. Level 0 is the image; level i has max(size >> 1, 1) per axis of level i - 1, each texel the
+ * largest of the 2 x 2 texels under it, plus the third column / row where the level above has an odd
+ * width / height (texels past its edge read 0); myvk's QueryMipLevel gives the level count (the bit
+ * length of width | height). The context keeps it until the next call. */
+int hcm_ctx_set_depth(hcm_ctx *ctx, const float *depth, uint32_t width, uint32_t height);
+/* Level `level` of that hierarchy (for checks): its size, and its texels into out when out is not NULL
+ * (HCM_ERR_CAPACITY when they do not fit cap floats). */
+int hcm_ctx_depth_level(hcm_ctx *ctx, uint32_t level, float *out, uint64_t cap, uint32_t *width, uint32_t *height);
+/* hcm_batch_cull plus the shader's occlusion test (chunk_culling.comp:84-90, 104) against the context's
+ * depth hierarchy, whose levels 1 .. n - 1 are the shader's texture mips 0 .. n - 2: a mesh whose corners
+ * all have w > 0 is culled when the texel nearest the centre of its clip-space box, at mip
+ * ceil(log2(max box extent in uv * max(mip 0 width, height))) clamped to the mips, is less than the box's
+ * nearest depth. Clip coordinates use IEEE division and the mip the exact binary logarithm (the shader's
+ * own division and log2 are approximate on GPUs). HCM_ERR_INVALID without a depth hierarchy. */
+int hcm_batch_cull_occlusion(hcm_batch *batch, const float view_position[3], const float view_projection[16],
+                             hcm_draw_cmd *opaque, uint32_t opaque_cap, uint32_t *n_opaque, hcm_draw_cmd *transparent,
+                             uint32_t transparent_cap, uint32_t *n_transparent);
 
 /* Launch shape of the meshing kernel. 0 = default. */
 typedef struct hcm_tuning {

@@ -48,3 +48,83 @@ def test_mesh_infos_and_culling_match_oracle(ctx):
     finally:
         batch.close()
         store.close()
+
+
+def test_depth_hierarchy_matches_oracle(ctx):
+    rng = np.random.default_rng(5)
+    for h, w in ((1080, 1920), (777, 333), (5, 7), (1, 3), (2, 1), (9, 1000)):
+        say(f"depth hierarchy {w} x {h}")
+        img = rng.random((h, w), dtype=np.float32)
+        ctx.set_depth(img)
+        want = do.depth_pyramid(img)
+        for L, lv in enumerate(want):
+            assert np.array_equal(ctx.depth_level(L), lv), f"{w} x {h} level {L}"
+        with pytest.raises(hcm.HcmError):
+            ctx.depth_level(len(want))
+
+
+def test_occlusion_culling_matches_oracle(ctx):
+    blocks, light, nbr = workloads.cave_region()
+    pos = workloads.region_job_positions()
+    store = hcm.Store(ctx, blocks.shape[0])
+    store.upload(blocks, light)
+    batch = hcm.Batch(ctx, store, nbr)
+    try:
+        batch.run()
+        batch.sync()
+        _, recs, _, _ = batch.download()
+        batch.mesh_infos(pos)
+        want = do.mesh_infos(recs, pos)
+        rng = np.random.default_rng(31)
+        yy, xx = np.mgrid[0:270, 0:480]
+        hidden = 0
+        for k in range(6):
+            eye = rng.uniform(-40, 300, 3).astype(np.float32)
+            vp = do.look_at_projection(eye, rng.uniform(0, 256, 3))
+            # a depth image around the median nearest depth of the meshes in view, so that some are hidden
+            bits, cmin, _ = do.clip_info(want["aabb"], eye, vp)
+            seen = ((bits & 127) == 0) & ((bits & 128) != 0)
+            if not seen.any():
+                continue
+            c = float(np.median(cmin[seen, 2]))
+            img = np.clip(c + 0.02 * (1 - c) * np.sin(xx / (5.0 + k)) * np.cos(yy / 3.0), 0, 1).astype(np.float32)
+            ctx.set_depth(img)
+            say(f"occlusion camera {k}: eye {eye.tolist()}, depth around {c:.5f}")
+            o, t = batch.cull(eye, vp, occlusion=True)
+            wo, wt = do.cull(want, eye, vp, do.depth_pyramid(img))
+            for g, w in ((o, wo), (t, wt)):
+                g = np.stack([g[f] for f in hcm.DRAW_CMD_DTYPE.names], 1)
+                assert np.array_equal(g[np.argsort(g[:, 4])], w), "draw commands"
+            fo, ft = batch.cull(eye, vp)
+            assert len(o) + len(t) <= len(fo) + len(ft)
+            hidden += len(fo) + len(ft) - len(o) - len(t)
+        assert hidden > 0
+    finally:
+        batch.close()
+        store.close()
+
+
+def test_occlusion_needs_a_depth_hierarchy():
+    c = hcm.Context(pool_quads=1 << 16)
+    try:
+        blocks, light, nbr = workloads.terrain_chunk()
+        store = hcm.Store(c, blocks.shape[0])
+        store.upload(blocks, light)
+        batch = hcm.Batch(c, store, nbr)
+        batch.run()
+        batch.sync()
+        batch.mesh_infos(np.zeros((nbr.shape[0], 3), np.int32))
+        eye = np.zeros(3, np.float32)
+        vp = do.look_at_projection(eye, (8.0, 8.0, 8.0))
+        with pytest.raises(hcm.HcmError):
+            batch.cull(eye, vp, occlusion=True)
+        with pytest.raises(hcm.HcmError):
+            c.set_depth(np.zeros((1, 1), np.float32))  # one texel: no hierarchy level
+        c.set_depth(np.ones((2, 2), np.float32))
+        o, t = batch.cull(eye, vp, occlusion=True)
+        fo, ft = batch.cull(eye, vp)
+        assert len(o) == len(fo) and len(t) == len(ft)  # the far plane hides nothing
+        batch.close()
+        store.close()
+    finally:
+        c.close()
