@@ -127,12 +127,10 @@ def relight_leg(ctx, hcm, workloads, meshdigest, steps, cpu_threads, with_cpu):
 
 
 def draw_leg(hcm, workloads, batch, n_meshes):
-    """The draw list after meshing: mesh infos of the cave batch's meshes (WorldRenderer::PushChunkMesh) and
-    frustum culling into indirect draws (chunk_culling.comp) for 20 seeded cameras; wall time per call,
-    host copies of the lists included."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import draw_oracle
-
+    """The draw list after meshing: mesh infos of the cave batch's meshes (WorldRenderer::PushChunkMesh),
+    frustum culling into indirect draws (chunk_culling.comp) for 20 seeded cameras, and the depth hierarchy
+    of a 1920 x 1080 depth image plus occlusion culling for the same cameras; wall time per call, host
+    copies of the lists (and of the depth image) included."""
     pos = workloads.region_job_positions()
     batch.mesh_infos(pos)
     t0 = time.perf_counter()
@@ -143,15 +141,35 @@ def draw_leg(hcm, workloads, batch, n_meshes):
     cams = []
     for _ in range(20):
         eye = rng.uniform(-40, 300, 3).astype(np.float32)
-        cams.append((eye, draw_oracle.look_at_projection(eye, rng.uniform(0, 256, 3))))
+        target = rng.uniform(0, 256, 3)
+        cams.append((eye, target, workloads.look_at_projection(eye, target)))
     kept = 0
     t0 = time.perf_counter()
-    for eye, vp in cams:
+    for eye, _, vp in cams:
         o, t = batch.cull(eye, vp)
         kept += len(o) + len(t)
     t_cull = (time.perf_counter() - t0) / len(cams)
+    # occlusion: per camera a depth image near the depth of the point 60 blocks ahead of the eye, rippled
+    yy, xx = np.mgrid[0:1080, 0:1920]
+    wave = (np.sin(xx / 37.0) * np.cos(yy / 23.0)).astype(np.float32)
+    kept_occ, t_depth, t_occ = 0, 0.0, 0.0
+    for eye, target, vp in cams:
+        ahead = 60.0 * (target - eye) / np.linalg.norm(target - eye)
+        clip = vp.astype(np.float64).reshape(4, 4).T @ np.append(ahead, 1.0)
+        c = np.float32(clip[2] / clip[3])
+        img = np.clip(c + np.float32(0.5) * (np.float32(1) - c) * wave, 0, 1).astype(np.float32)
+        t0 = time.perf_counter()
+        batch.ctx.set_depth(img)
+        t1 = time.perf_counter()
+        o, t = batch.cull(eye, vp, occlusion=True)
+        t_occ += time.perf_counter() - t1
+        t_depth += t1 - t0
+        kept_occ += len(o) + len(t)
     return {"meshes": int(n_meshes), "infos": int(len(infos)), "mesh_infos_ms": round(t_info * 1e3, 4),
-            "cull_ms": round(t_cull * 1e3, 4), "kept_per_camera": round(kept / len(cams), 1)}
+            "cull_ms": round(t_cull * 1e3, 4), "kept_per_camera": round(kept / len(cams), 1),
+            "depth_hierarchy_1080p_ms": round(t_depth / len(cams) * 1e3, 4),
+            "cull_occlusion_ms": round(t_occ / len(cams) * 1e3, 4),
+            "kept_per_camera_occlusion": round(kept_occ / len(cams), 1)}
 
 
 def measured_traffic():

@@ -242,3 +242,25 @@ def slice_heights(heights, nbr, job):
 def region_job_positions(n=16):
     """Chunk coordinates (J, 3) int32 of the jobs of cave_region / lit_cave_region (region chunks 0..n-1)."""
     return (_grid_coords((1, 1, 1), (n, n, n)) - 1).astype(np.int32)
+
+
+# ---- cameras for the draw-list step ----
+def look_at_projection(eye, target, fov_y_deg=70.0, aspect=16 / 9, near=0.1, far=512.0):
+    """A camera for the draw-list workloads: the view-projection matrix (column-major, float32) looking
+    from eye to target, in the view-position relative form the culling shader uses (the view translation
+    is applied by subtracting the view position), with depth in [0, 1] between near and far."""
+    eye, target = np.asarray(eye, np.float64), np.asarray(target, np.float64)
+    f = target - eye
+    f /= np.linalg.norm(f)
+    up = np.array([0.0, 1.0, 0.0]) if abs(f[1]) < 0.99 else np.array([1.0, 0.0, 0.0])
+    s = np.cross(f, up)
+    s /= np.linalg.norm(s)
+    u = np.cross(s, f)
+    view = np.eye(4)
+    view[0, :3], view[1, :3], view[2, :3] = s, u, -f
+    t = 1.0 / np.tan(np.radians(fov_y_deg) / 2)
+    proj = np.zeros((4, 4))
+    proj[0, 0], proj[1, 1] = t / aspect, t
+    proj[2, 2], proj[2, 3] = far / (near - far), far * near / (near - far)
+    proj[3, 2] = -1.0
+    return (proj @ view).T.astype(np.float32).reshape(16)  # column-major

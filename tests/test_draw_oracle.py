@@ -7,7 +7,7 @@ of the float32 decisions with a float64 evaluation for every mesh not within rou
 import numpy as np
 
 import draw_oracle as do
-from hypercraft_amd import hcm
+from hypercraft_amd import hcm, workloads
 
 
 def rec(aabb, first_quad=0, quad_count=1, job=0, transparent=0):
@@ -34,7 +34,7 @@ def boxes(*aabbs):
 
 def test_frustum_closed_forms():
     eye = np.array([0.0, 0.0, 0.0], np.float32)
-    vp = do.look_at_projection(eye, (0.0, 0.0, -1.0))  # looking down -z
+    vp = workloads.look_at_projection(eye, (0.0, 0.0, -1.0))  # looking down -z
     infos = boxes((-1, -1, -12, 1, 1, -10),      # in front: kept
                   (-1, -1, 10, 1, 1, 12),        # behind the eye: every w <= 0
                   (-200, -1, -12, -150, 1, -10), # far to the left of the view
@@ -53,7 +53,7 @@ def test_float32_decisions_agree_with_float64():
     aabb = np.concatenate([lo, lo + rng.uniform(0.0625, 16, (4000, 3))], 1).astype(np.float32)
     for _ in range(10):
         eye = rng.uniform(-100, 100, 3).astype(np.float32)
-        vp = do.look_at_projection(eye, eye + rng.normal(size=3))
+        vp = workloads.look_at_projection(eye, eye + rng.normal(size=3))
         keep32 = do.keep_mask(aabb, eye, vp)
         # float64: the same test, with a margin: corners within 1e-3 of a plane are left out
         m = vp.astype(np.float64).reshape(4, 4).T
@@ -112,7 +112,7 @@ def test_ceil_log2_closed_forms():
 
 def wall_scene():
     eye = np.array([0.0, 0.0, 0.0], np.float32)
-    vp = do.look_at_projection(eye, (0.0, 0.0, -1.0))
+    vp = workloads.look_at_projection(eye, (0.0, 0.0, -1.0))
     zc = lambda d: float(do.project(vp, np.float32(0), np.float32(0), np.float32(-d))[2] /
                          do.project(vp, np.float32(0), np.float32(0), np.float32(-d))[3])
     # a 128 x 72 depth image: a wall at distance 20 over the middle third, the far plane elsewhere
@@ -151,7 +151,7 @@ def test_occlusion_float32_agrees_with_float64():
     checked = 0
     for k in range(6):
         eye = (rng.uniform(-20, 20, 3) + np.array([0, 0, 90])).astype(np.float32)
-        vp = do.look_at_projection(eye, (0.0, 0.0, 0.0))
+        vp = workloads.look_at_projection(eye, (0.0, 0.0, 0.0))
         img = (0.985 + 0.015 * np.sin(xx / (7 + k)) * np.cos(yy / 5)).astype(np.float32)
         p = do.depth_pyramid(img)
         bits, cmin, cmax = do.clip_info(aabb, eye, vp)

@@ -37,7 +37,7 @@ def test_mesh_infos_and_culling_match_oracle(ctx):
         rng = np.random.default_rng(12)
         for k in range(6):
             eye = rng.uniform(-40, 300, 3).astype(np.float32)
-            vp = do.look_at_projection(eye, rng.uniform(0, 256, 3), fov_y_deg=float(rng.uniform(40, 100)))
+            vp = workloads.look_at_projection(eye, rng.uniform(0, 256, 3), fov_y_deg=float(rng.uniform(40, 100)))
             say(f"camera {k}: eye {eye.tolist()}")
             o, t = batch.cull(eye, vp)
             wo, wt = do.cull(want, eye, vp)
@@ -80,7 +80,7 @@ def test_occlusion_culling_matches_oracle(ctx):
         hidden = 0
         for k in range(6):
             eye = rng.uniform(-40, 300, 3).astype(np.float32)
-            vp = do.look_at_projection(eye, rng.uniform(0, 256, 3))
+            vp = workloads.look_at_projection(eye, rng.uniform(0, 256, 3))
             # a depth image around the median nearest depth of the meshes in view, so that some are hidden
             bits, cmin, _ = do.clip_info(want["aabb"], eye, vp)
             seen = ((bits & 127) == 0) & ((bits & 128) != 0)
@@ -115,7 +115,7 @@ def test_occlusion_needs_a_depth_hierarchy():
         batch.sync()
         batch.mesh_infos(np.zeros((nbr.shape[0], 3), np.int32))
         eye = np.zeros(3, np.float32)
-        vp = do.look_at_projection(eye, (8.0, 8.0, 8.0))
+        vp = workloads.look_at_projection(eye, (8.0, 8.0, 8.0))
         with pytest.raises(hcm.HcmError):
             batch.cull(eye, vp, occlusion=True)
         with pytest.raises(hcm.HcmError):
