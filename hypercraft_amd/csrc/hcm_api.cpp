@@ -805,7 +805,7 @@ static hipError_t enqueue_light(hcm_batch *b, hipStream_t st) {
 	if (e == hipSuccess)
 		e = launch_light_prep(lp, st);
 	if (e == hipSuccess) {
-		const int per_cu = std::max(1, std::min(2, (160 * 1024) / std::max(1, light_lds_bytes())));
+		const int per_cu = std::max(1, std::min(3, (160 * 1024) / std::max(1, light_lds_bytes())));
 		e = launch_light(lp, per_cu * b->ctx->n_cu, st);
 	}
 	return e;

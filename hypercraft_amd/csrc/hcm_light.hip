@@ -19,9 +19,10 @@
 //   light_prep_kernel  one workgroup per store slot the batch reads: packs its blocks' light pass, its
 //                      sunlit cells (from the sunlight heights), its torchlight levels and its level-15
 //                      torchlight sources into 16-bit x-rows (3.5 KB per slot);
-//   light_kernel       one workgroup per job (persistent, per-XCD work queues): assembles the 46^3 window
-//                      from the 27 slots' packed rows in LDS, runs the sunlight and torchlight levels,
-//                      and writes the 18^3 rebuilt light bytes (sun | torch << 4) the mesher reads.
+//   light_kernel       one workgroup per job (persistent, per-XCD work queues, three per CU): assembles
+//                      the 46^3 window from the 27 slots' packed rows in LDS, runs the sunlight and
+//                      torchlight levels, and writes the 18^3 rebuilt light bytes (sun | torch << 4) the
+//                      mesher reads.
 #include <hip/hip_runtime.h>
 
 #include "hcm_internal.h"
@@ -37,12 +38,15 @@ constexpr int kHRows = kH * kH;        // halo x-rows, index hz + 18 hy
 constexpr int kHOff = 14;              // window index of halo index 0
 constexpr uint64_t kWMask = (1ull << kW) - 1ull;
 constexpr int kLightThreads = 512;
+constexpr int kIn = kW - 2;          // the level steps update rows y, z in [-14, 30): window rows 1 .. 44
+constexpr int kInRows = kIn * kIn;
 
+// 53.4 KB: three workgroups per CU
 struct LightSmem {
-	uint64_t pass[kWRows];    // cells that pass light indirectly (Block::GetIndirectLightPass, Block.hpp:70)
-	uint64_t g[2][kWRows];    // G_{k+1} and G_k
-	uint64_t t15[kWRows];     // torchlight sources of level 15, kept for the torchlight channel
-	uint32_t lv[2][4][kHRows]; // per channel (sun, torch): bit planes of the level of each halo cell
+	uint64_t pass[kInRows]; // cells that pass light indirectly (Block::GetIndirectLightPass, Block.hpp:70)
+	uint64_t g[2][kWRows];  // G_{k+1} and G_k
+	uint64_t lv012[kHRows]; // the channel's level of each halo cell: bit planes 0, 1, 2 at bits 0, 18, 36
+	uint32_t lv3[kHRows];   // and bit plane 3
 	uint32_t slots[27];
 	uint32_t changed[3]; // per level step (rotating): some row of the step changed
 	uint32_t levels;     // torchlight levels present in the 27 chunks
@@ -82,20 +86,19 @@ __device__ __forceinline__ uint64_t level_row(const RowSrc &r, uint32_t k) {
 	return m;
 }
 
-// One channel: G_15 is in s.g[0]; records each halo cell's level in s.lv[ch]. A step that adds no cell
+// One channel: G_15 is in s.g[0]; records each halo cell's level in s.lv012 / s.lv3. A step that adds no cell
 // to its rows leaves G unchanged for every later step (each later step reads only rows of the earlier
 // step), so the channel ends there unless sources enter at a lower level.
 template <int NT>
-__device__ __forceinline__ void propagate(const LightParams &p, LightSmem &s, int ch, bool inject) {
+__device__ __forceinline__ void propagate(const LightParams &p, LightSmem &s, bool inject) {
 	const int tid = threadIdx.x;
 	if (tid < 3)
 		s.changed[tid] = 0;
 	for (int hr = tid; hr < kHRows; hr += NT) {
 		const int hy = hr / kH, hz = hr % kH;
-		const uint32_t bits = uint32_t(s.g[0][(hz + kHOff) + kW * (hy + kHOff)] >> kHOff) & 0x3ffffu;
-#pragma unroll
-		for (int b = 0; b < 4; ++b)
-			s.lv[ch][b][hr] = bits; // level 15
+		const uint64_t bits = (s.g[0][(hz + kHOff) + kW * (hy + kHOff)] >> kHOff) & 0x3ffffu;
+		s.lv012[hr] = bits | (bits << 18) | (bits << 36); // level 15
+		s.lv3[hr] = uint32_t(bits);
 	}
 	__syncthreads();
 	int cur = 0;
@@ -114,20 +117,21 @@ __device__ __forceinline__ void propagate(const LightParams &p, LightSmem &s, in
 			const int r = wz + kW * wy;
 			const uint64_t g = gi[r];
 			const uint64_t d = (g << 1) | (g >> 1) | gi[r - 1] | gi[r + 1] | gi[r - kW] | gi[r + kW];
-			uint64_t nw = g | (d & s.pass[r]);
+			uint64_t nw = g | (d & s.pass[(wz - 1) + kIn * (wy - 1)]);
 			if (inj)
 				nw |= level_row(row_src(p, s, wy, wz), k);
 			go[r] = nw;
 			changed |= nw != g;
 			const int hy = wy - kHOff, hz = wz - kHOff;
 			if ((unsigned)hy < (unsigned)kH && (unsigned)hz < (unsigned)kH) {
-				const uint32_t add = uint32_t((nw & ~g) >> kHOff) & 0x3ffffu;
+				const uint64_t add = ((nw & ~g) >> kHOff) & 0x3ffffu;
 				if (add) {
 					const int hr = hz + kH * hy;
-#pragma unroll
-					for (int b = 0; b < 4; ++b)
-						if ((k >> b) & 1u)
-							s.lv[ch][b][hr] |= add;
+					const uint64_t m = ((k & 1u) ? add : 0ull) | ((k & 2u) ? add << 18 : 0ull) | ((k & 4u) ? add << 36 : 0ull);
+					if (m)
+						s.lv012[hr] |= m;
+					if (k & 8u)
+						s.lv3[hr] |= uint32_t(add);
 				}
 			}
 		}
@@ -156,39 +160,45 @@ template <int NT> __device__ __forceinline__ void light_job(const LightParams &p
 	__syncthreads();
 	const bool torch = s.levels != 0, torch15 = (s.levels >> 15) & 1u;
 	for (int r = tid; r < kWRows; r += NT) {
-		const RowSrc src = row_src(p, s, r / kW, r % kW);
-		s.pass[r] = plane_row(src, 0);
+		const int wy = r / kW, wz = r % kW;
+		const RowSrc src = row_src(p, s, wy, wz);
+		if (unsigned(wy - 1) < unsigned(kIn) && unsigned(wz - 1) < unsigned(kIn))
+			s.pass[(wz - 1) + kIn * (wy - 1)] = plane_row(src, 0);
 		s.g[0][r] = plane_row(src, 1); // sunlight: 15 at and above the sunlight height (Chunk.hpp:73-79)
-		s.t15[r] = torch15 ? plane_row(src, 6) : 0ull;
 	}
 	__syncthreads();
-	propagate<NT>(p, s, 0, false);
+	// light bytes sun | torch << 4 (block/include/block/Light.hpp), one 20-byte record per halo x-row:
+	// the sunlight nibbles first, then the torchlight nibbles into the same records (each thread rewrites
+	// only the records it wrote)
+	uint32_t *out = reinterpret_cast<uint32_t *>(p.halo_light + (size_t)job * kLightJobBytes);
+	auto put = [&](int shift) {
+		for (int hr = tid; hr < kHRows; hr += NT) {
+			const uint64_t l012 = s.lv012[hr];
+			const uint32_t pl[4] = {uint32_t(l012) & 0x3ffffu, uint32_t(l012 >> 18) & 0x3ffffu,
+			                        uint32_t(l012 >> 36) & 0x3ffffu, s.lv3[hr]};
+			uint32_t w[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+			for (int x = 0; x < kH; ++x) {
+				uint32_t v = 0;
+#pragma unroll
+				for (int b = 0; b < 4; ++b)
+					v |= ((pl[b] >> x) & 1u) << b;
+				w[x >> 2] |= v << (8 * (x & 3) + shift);
+			}
+			uint32_t *o = out + hr * (kLightRowBytes / 4);
+#pragma unroll
+			for (int q = 0; q < 5; ++q)
+				o[q] = shift ? (o[q] | w[q]) : w[q];
+		}
+	};
+	propagate<NT>(p, s, false);
+	put(0);
 	if (torch) {
 		for (int r = tid; r < kWRows; r += NT)
-			s.g[0][r] = s.t15[r];
+			s.g[0][r] = torch15 ? plane_row(row_src(p, s, r / kW, r % kW), 6) : 0ull;
 		__syncthreads();
-		propagate<NT>(p, s, 1, true);
-	}
-	// light bytes sun | torch << 4 (block/include/block/Light.hpp), one halo x-row per thread
-	uint8_t *out = p.halo_light + (size_t)job * kLightJobBytes;
-	for (int hr = tid; hr < kHRows; hr += NT) {
-		uint32_t sp[4], tp[4];
-#pragma unroll
-		for (int b = 0; b < 4; ++b)
-			sp[b] = s.lv[0][b][hr], tp[b] = torch ? s.lv[1][b][hr] : 0u;
-		uint32_t w[5] = {0, 0, 0, 0, 0};
-#pragma unroll
-		for (int x = 0; x < kH; ++x) {
-			uint32_t v = 0;
-#pragma unroll
-			for (int b = 0; b < 4; ++b)
-				v |= ((sp[b] >> x) & 1u) << b | ((tp[b] >> x) & 1u) << (4 + b);
-			w[x >> 2] |= v << (8 * (x & 3));
-		}
-		uint32_t *o = reinterpret_cast<uint32_t *>(out + hr * kLightRowBytes);
-#pragma unroll
-		for (int q = 0; q < 5; ++q)
-			o[q] = w[q];
+		propagate<NT>(p, s, true);
+		put(4);
 	}
 }
 
@@ -236,7 +246,7 @@ __global__ __launch_bounds__(256) void light_prep_kernel(LightParams p) {
 		p.levels[slot] = s_levels;
 }
 
-template <int NT> __global__ __launch_bounds__(NT, 2) void light_kernel(LightParams p) {
+template <int NT> __global__ __launch_bounds__(NT, 3) void light_kernel(LightParams p) {
 	__shared__ LightSmem s;
 	// part q of the job list is [q * n / kQueues, (q + 1) * n / kQueues): a workgroup takes jobs from
 	// its own XCD's part first, then from the others

@@ -499,7 +499,7 @@ struct hcm_server {
 				lp.halo_light = b.d_halo;
 				HCM_CHECK(launch_reset(b.d_lctr, s_comp));
 				HCM_CHECK(launch_light_prep(lp, s_comp));
-				HCM_CHECK(launch_light(lp, std::max(1, std::min(info.n_cu * 2, (int)b.jobs.size())), s_comp));
+				HCM_CHECK(launch_light(lp, std::max(1, std::min(info.n_cu * 3, (int)b.jobs.size())), s_comp));
 			}
 			HCM_CHECK(launch_reset(b.d_ctr, s_comp));
 			MeshParams p;
