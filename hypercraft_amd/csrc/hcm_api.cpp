@@ -169,6 +169,10 @@ constexpr uint32_t kPlainPoolQuads = 1u << 17; // > the most quads one 16^3 chun
 static_assert(sizeof(hcm_mesh_info) == 52, "chunk_culling.comp MeshInfo layout (std430)");
 static_assert(sizeof(hcm_draw_cmd) == 20, "VkDrawIndexedIndirectCommand layout");
 constexpr uint32_t kPlainRecs = 64;
+// Workgroup size of launches that mesh one chunk (hcm_mesh_chunk, hcm_mesh_chunk_relight, HCM_RUN_PER_CHUNK):
+// a lone workgroup has the GPU to itself, so more lanes per chunk shorten it (cave chunks: 23.9 us at 512
+// threads, 27.7 us at 256, 37.3 us at 128). The persistent batch keeps its tuned shape.
+constexpr int kChunkLaunchThreads = 512;
 
 } // namespace
 
@@ -528,7 +532,7 @@ extern "C" int hcm_mesh_chunk(hcm_ctx *ctx, const uint16_t *const blocks[27], co
 	p.recs = ctx->d_precs;
 	p.rec_cap = kPlainRecs;
 	p.results = ctx->d_pres;
-	HCM_HIP(launch_mesh(p, (int)ctx->tuning.threads, 1, st));
+	HCM_HIP(launch_mesh(p, kChunkLaunchThreads, 1, st));
 	hcm_job_result res;
 	HCM_HIP(hipMemcpyAsync(&res, ctx->d_pres, sizeof(res), hipMemcpyDeviceToHost, st));
 	HCM_HIP(hipStreamSynchronize(st));
@@ -584,7 +588,7 @@ extern "C" int hcm_mesh_chunk_relight(hcm_ctx *ctx, const uint16_t *const blocks
 	p.recs = ctx->d_precs;
 	p.rec_cap = kPlainRecs;
 	p.results = ctx->d_pres;
-	HCM_HIP(launch_mesh(p, (int)ctx->tuning.threads, 1, st));
+	HCM_HIP(launch_mesh(p, kChunkLaunchThreads, 1, st));
 	hcm_job_result res;
 	HCM_HIP(hipMemcpyAsync(&res, ctx->d_pres, sizeof(res), hipMemcpyDeviceToHost, st));
 	std::vector<uint8_t> rows;
@@ -855,7 +859,7 @@ extern "C" int hcm_batch_run(hcm_batch *b, uint32_t mode) {
 		p.persistent = 0;
 		for (uint32_t j = 0; j < b->n_jobs; ++j) {
 			p.job_base = j;
-			HCM_HIP(launch_mesh(p, (int)threads, 1, st));
+			HCM_HIP(launch_mesh(p, kChunkLaunchThreads, 1, st));
 		}
 		HCM_HIP(hipEventRecord(b->ev1, st));
 	}
