@@ -86,9 +86,9 @@ __device__ __forceinline__ uint64_t level_row(const RowSrc &r, uint32_t k) {
 	return m;
 }
 
-// One channel: G_15 is in s.g[0]; records each halo cell's level in s.lv012 / s.lv3. A step that adds no cell
-// to its rows leaves G unchanged for every later step (each later step reads only rows of the earlier
-// step), so the channel ends there unless sources enter at a lower level.
+// One channel: G_15 is in s.g[0]; records each halo cell's level in s.lv012 / s.lv3. A step that adds
+// no cell to its rows leaves G unchanged for every later step (each later step reads only rows of the
+// earlier step), so the channel ends there unless sources enter at a lower level.
 template <int NT>
 __device__ __forceinline__ void propagate(const LightParams &p, LightSmem &s, bool inject) {
 	const int tid = threadIdx.x;
@@ -127,7 +127,8 @@ __device__ __forceinline__ void propagate(const LightParams &p, LightSmem &s, bo
 				const uint64_t add = ((nw & ~g) >> kHOff) & 0x3ffffu;
 				if (add) {
 					const int hr = hz + kH * hy;
-					const uint64_t m = ((k & 1u) ? add : 0ull) | ((k & 2u) ? add << 18 : 0ull) | ((k & 4u) ? add << 36 : 0ull);
+					const uint64_t m =
+					    ((k & 1u) ? add : 0ull) | ((k & 2u) ? add << 18 : 0ull) | ((k & 4u) ? add << 36 : 0ull);
 					if (m)
 						s.lv012[hr] |= m;
 					if (k & 8u)
