@@ -76,9 +76,10 @@ def test_occlusion_culling_matches_oracle(ctx):
         batch.mesh_infos(pos)
         want = do.mesh_infos(recs, pos)
         rng = np.random.default_rng(31)
-        yy, xx = np.mgrid[0:270, 0:480]
         hidden = 0
-        for k in range(6):
+        for k in range(8):
+            # even and odd image sizes: odd levels take a third column / row of texels
+            yy, xx = np.mgrid[0:270, 0:480] if k % 2 == 0 else np.mgrid[0:187, 0:333]
             eye = rng.uniform(-40, 300, 3).astype(np.float32)
             vp = workloads.look_at_projection(eye, rng.uniform(0, 256, 3))
             # a depth image around the median nearest depth of the meshes in view, so that some are hidden
