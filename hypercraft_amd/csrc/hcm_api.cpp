@@ -245,6 +245,7 @@ struct hcm_batch {
 	uint32_t graph_threads = 0, graph_grid = 0;
 	uint32_t last_mode = 0;
 	bool ran = false;
+	bool ctr_clean = false; // the counters are zero: a persistent run needs no reset kernel
 	// light rebuild (HCM_LIGHT_REBUILD)
 	uint32_t light_mode = HCM_LIGHT_STORE, graph_light = HCM_LIGHT_STORE;
 	uint32_t n_read_slots = 0;
@@ -693,8 +694,10 @@ extern "C" int hcm_batch_create(hcm_ctx *ctx, hcm_store *store, const uint32_t *
 	    dmalloc(&b->d_recs, b->rec_cap) != hipSuccess || dmalloc(&b->d_res, n_jobs) != hipSuccess ||
 	    hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess)
 		return fail();
-	if (hipMemcpy(b->d_nbr, nbr, 27ull * n_jobs * 4, hipMemcpyHostToDevice) != hipSuccess)
+	if (hipMemcpy(b->d_nbr, nbr, 27ull * n_jobs * 4, hipMemcpyHostToDevice) != hipSuccess ||
+	    hipMemset(b->d_ctr, 0, sizeof(Counters)) != hipSuccess)
 		return fail();
+	b->ctr_clean = true;
 	// the distinct store slots the jobs read, for the light rebuild's packing pass
 	std::vector<uint32_t> slots(nbr, nbr + 27ull * n_jobs);
 	std::sort(slots.begin(), slots.end());
@@ -833,10 +836,9 @@ extern "C" int hcm_batch_run(hcm_batch *b, uint32_t mode) {
 			if (b->graph)
 				(void)hipGraphDestroy(b->graph), b->graph = nullptr;
 			p.persistent = 1;
+			p.self_reset = 1; // the counters start at zero and the run's last workgroup zeroes them again
 			HCM_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
 			hipError_t e1 = b->light_mode == HCM_LIGHT_REBUILD ? enqueue_light(b, st) : hipSuccess;
-			if (e1 == hipSuccess)
-				e1 = launch_reset(b->d_ctr, st);
 			hipError_t e2 = e1 == hipSuccess ? launch_mesh(p, (int)threads, (int)grid, st) : hipSuccess;
 			hipError_t e3 = hipStreamEndCapture(st, &b->graph);
 			if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
@@ -848,9 +850,12 @@ extern "C" int hcm_batch_run(hcm_batch *b, uint32_t mode) {
 			b->graph_grid = grid;
 			b->graph_light = b->light_mode;
 		}
+		if (!b->ctr_clean) // after per-chunk launches
+			HCM_HIP(launch_reset(b->d_ctr, st));
 		HCM_HIP(hipEventRecord(b->ev0, st));
 		HCM_HIP(hipGraphLaunch(b->graph_exec, st));
 		HCM_HIP(hipEventRecord(b->ev1, st));
+		b->ctr_clean = true;
 	} else {
 		HCM_HIP(hipEventRecord(b->ev0, st));
 		if (b->light_mode == HCM_LIGHT_REBUILD)
@@ -862,6 +867,7 @@ extern "C" int hcm_batch_run(hcm_batch *b, uint32_t mode) {
 			HCM_HIP(launch_mesh(p, kChunkLaunchThreads, 1, st));
 		}
 		HCM_HIP(hipEventRecord(b->ev1, st));
+		b->ctr_clean = false;
 	}
 	b->ran = true;
 	b->infos_ready = false;
@@ -888,17 +894,31 @@ extern "C" int hcm_batch_results(hcm_batch *b, hcm_job_result *results) {
 	return HCM_OK;
 }
 
+// The quads and mesh records of the batch's last run: a persistent run's last workgroup copies them to
+// total_quads / total_meshes before it clears the cursors; per-chunk launches leave them in the cursors.
+static hipError_t run_totals(hcm_batch *b, uint32_t *quads, uint32_t *meshes) {
+	Counters ctr;
+	hipStream_t st = b->ctx->stream;
+	hipError_t e = hipMemcpyAsync(&ctr, b->d_ctr, sizeof(ctr), hipMemcpyDeviceToHost, st);
+	if (e == hipSuccess)
+		e = hipStreamSynchronize(st);
+	if (e != hipSuccess)
+		return e;
+	const bool persistent = b->last_mode == HCM_RUN_PERSISTENT;
+	*quads = persistent ? ctr.total_quads : ctr.pool_cursor;
+	*meshes = persistent ? ctr.total_meshes : ctr.mesh_cursor;
+	return hipSuccess;
+}
+
 extern "C" int hcm_batch_totals(hcm_batch *b, uint64_t *quads, uint64_t *meshes) {
 	if (!b || !b->ran)
 		return HCM_ERR_INVALID;
-	Counters ctr;
-	hipStream_t st = b->ctx->stream;
-	HCM_HIP(hipMemcpyAsync(&ctr, b->d_ctr, sizeof(ctr), hipMemcpyDeviceToHost, st));
-	HCM_HIP(hipStreamSynchronize(st));
+	uint32_t nq, nm;
+	HCM_HIP(run_totals(b, &nq, &nm));
 	if (quads)
-		*quads = ctr.pool_cursor;
+		*quads = nq;
 	if (meshes)
-		*meshes = ctr.mesh_cursor;
+		*meshes = nm;
 	return HCM_OK;
 }
 
@@ -925,11 +945,10 @@ extern "C" int hcm_batch_mesh_infos(hcm_batch *b, const int32_t *chunk_pos, hcm_
 		HCM_HIP(dmalloc(&b->d_draws, 2ull * b->rec_cap));
 		HCM_HIP(dmalloc(&b->d_draw_counts, 2));
 	}
-	Counters ctr;
-	HCM_HIP(hipMemcpyAsync(&ctr, b->d_ctr, sizeof(ctr), hipMemcpyDeviceToHost, st));
+	uint32_t total_quads, total_meshes;
+	HCM_HIP(run_totals(b, &total_quads, &total_meshes));
 	HCM_HIP(hipMemcpyAsync(b->d_chunk_pos, chunk_pos, 12ull * b->n_jobs, hipMemcpyHostToDevice, st));
-	HCM_HIP(hipStreamSynchronize(st));
-	const uint32_t nr = std::min<uint32_t>(ctr.mesh_cursor, b->rec_cap);
+	const uint32_t nr = std::min<uint32_t>(total_meshes, b->rec_cap);
 	HCM_HIP(launch_mesh_infos(b->d_recs, nr, b->d_chunk_pos, b->d_infos, st));
 	*n = nr;
 	b->n_infos = nr;
@@ -1056,11 +1075,10 @@ extern "C" int hcm_batch_download(hcm_batch *b, hcm_job_result *results, hcm_mes
 	if (!b || !b->ran)
 		return HCM_ERR_INVALID;
 	hipStream_t st = b->ctx->stream;
-	Counters ctr;
-	HCM_HIP(hipMemcpyAsync(&ctr, b->d_ctr, sizeof(ctr), hipMemcpyDeviceToHost, st));
-	HCM_HIP(hipStreamSynchronize(st));
-	const uint64_t nq = std::min<uint64_t>(ctr.pool_cursor, b->ctx->pool_quads);
-	const uint32_t nr = std::min<uint32_t>(ctr.mesh_cursor, b->rec_cap);
+	uint32_t total_quads, total_meshes;
+	HCM_HIP(run_totals(b, &total_quads, &total_meshes));
+	const uint64_t nq = std::min<uint64_t>(total_quads, b->ctx->pool_quads);
+	const uint32_t nr = std::min<uint32_t>(total_meshes, b->rec_cap);
 	if (results)
 		HCM_HIP(hipMemcpyAsync(results, b->d_res, (size_t)b->n_jobs * sizeof(hcm_job_result), hipMemcpyDeviceToHost,
 		                       st));

@@ -54,6 +54,10 @@ struct Counters {
 	Line queue[kQueues];               // jobs taken from each part of the job list
 	alignas(256) uint32_t pool_cursor; // quads allocated in the pool
 	alignas(256) uint32_t mesh_cursor; // mesh records allocated
+	// self-resetting persistent runs: workgroups done; the last one copies the run's totals below and
+	// clears every counter for the next run
+	alignas(256) uint32_t done;
+	uint32_t total_quads, total_meshes;
 };
 
 struct MeshParams {
@@ -70,7 +74,7 @@ struct MeshParams {
 	uint16_t *pool_idx;  // 6 indices per quad
 	hcm_mesh_rec *recs;
 	uint32_t rec_cap;
-	uint32_t pad;
+	uint32_t self_reset; // persistent: the last workgroup to finish clears the counters (no reset kernel)
 	hcm_job_result *results;
 	// [n_jobs][kLightJobBytes]: each job's rebuilt light over its halo, or null: light from the store
 	const uint8_t *halo_light;

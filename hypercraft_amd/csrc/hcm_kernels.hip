@@ -1430,6 +1430,20 @@ template <int NT> __global__ __launch_bounds__(NT, NT >= 512 ? 2 : 3) void mesh_
 		mesh_chunk(p, s, job);
 		__syncthreads();
 	}
+	// every claim and allocation of this workgroup precedes its increment of `done`, so the last
+	// workgroup to increment it sees final totals and may clear the counters for the next run
+	if (p.self_reset && threadIdx.x == 0) {
+		__threadfence();
+		if (atomicAdd(&p.ctr->done, 1u) == gridDim.x - 1) {
+			p.ctr->total_quads = atomicAdd(&p.ctr->pool_cursor, 0u);
+			p.ctr->total_meshes = atomicAdd(&p.ctr->mesh_cursor, 0u);
+			for (uint32_t q = 0; q < kQueues; ++q)
+				p.ctr->queue[q].v = 0;
+			p.ctr->pool_cursor = 0;
+			p.ctr->mesh_cursor = 0;
+			p.ctr->done = 0;
+		}
+	}
 }
 
 __global__ void reset_kernel(Counters *ctr) {
@@ -1438,6 +1452,7 @@ __global__ void reset_kernel(Counters *ctr) {
 			ctr->queue[q].v = 0;
 		ctr->pool_cursor = 0;
 		ctr->mesh_cursor = 0;
+		ctr->done = 0;
 	}
 }
 
