@@ -246,6 +246,7 @@ struct hcm_batch {
 	uint32_t last_mode = 0;
 	bool ran = false;
 	bool ctr_clean = false; // the counters are zero: a persistent run needs no reset kernel
+	bool launched = false;  // a persistent run has been launched directly: the next one captures the graph
 	// light rebuild (HCM_LIGHT_REBUILD)
 	uint32_t light_mode = HCM_LIGHT_STORE, graph_light = HCM_LIGHT_STORE;
 	uint32_t n_read_slots = 0;
@@ -828,7 +829,21 @@ extern "C" int hcm_batch_run(hcm_batch *b, uint32_t mode) {
 	hipStream_t st = c->stream;
 	const uint32_t threads = c->tuning.threads;
 	MeshParams p = batch_params(b);
-	if (mode == HCM_RUN_PERSISTENT) {
+	if (mode == HCM_RUN_PERSISTENT && !b->launched) {
+		// first persistent run: launched directly, since capturing and instantiating a graph costs more
+		// than the launches it replaces; the graph is captured when the batch runs again
+		if (!b->ctr_clean)
+			HCM_HIP(launch_reset(b->d_ctr, st));
+		HCM_HIP(hipEventRecord(b->ev0, st));
+		if (b->light_mode == HCM_LIGHT_REBUILD)
+			HCM_HIP(enqueue_light(b, st));
+		p.persistent = 1;
+		p.self_reset = 1;
+		HCM_HIP(launch_mesh(p, (int)threads, persistent_grid(c, threads), st));
+		HCM_HIP(hipEventRecord(b->ev1, st));
+		b->ctr_clean = true;
+		b->launched = true;
+	} else if (mode == HCM_RUN_PERSISTENT) {
 		const uint32_t grid = (uint32_t)persistent_grid(c, threads);
 		if (!b->graph_exec || b->graph_threads != threads || b->graph_grid != grid || b->graph_light != b->light_mode) {
 			if (b->graph_exec)
