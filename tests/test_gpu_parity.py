@@ -156,6 +156,32 @@ def test_per_chunk_launches_match(ctx, golden):
     assert_digests("cave_region subset", digests_of(out), golden["cave_region_digest"][::37])
 
 
+def test_run_modes_in_any_order(ctx, golden):
+    """A batch's first persistent run (launched directly), its graph replays (whose last workgroup clears
+    the counters) and per-chunk launches (after which the counters are reset) give the same meshes and
+    totals in any order."""
+    blocks, light, nbr = gset("cave_region")
+    sub = np.arange(0, nbr.shape[0], 23)
+    want_d = golden["cave_region_digest"][sub]
+    want_q = int(golden["cave_region_quads"][sub].sum())
+    store = hcm.Store(ctx, blocks.shape[0])
+    store.upload(blocks, light)
+    batch = hcm.Batch(ctx, store, nbr[sub])
+    try:
+        for k, mode in enumerate((hcm.RUN_PERSISTENT, hcm.RUN_PERSISTENT, hcm.RUN_PER_CHUNK, hcm.RUN_PERSISTENT,
+                                  hcm.RUN_PER_CHUNK, hcm.RUN_PER_CHUNK, hcm.RUN_PERSISTENT, hcm.RUN_PERSISTENT)):
+            say(f"run {k}: mode {mode}")
+            batch.run(mode)
+            batch.sync()
+            quads, meshes = batch.totals()
+            out = batch.download()
+            assert quads == want_q and meshes == out[1].shape[0], f"run {k}"
+            assert_digests(f"run {k}", digests_of(out), want_d)
+    finally:
+        batch.close()
+        store.close()
+
+
 @pytest.mark.parametrize("threads", [128, 512])
 def test_workgroup_sizes_agree(ctx, golden, threads):
     blocks, light, nbr = gset("palette_all_values")
