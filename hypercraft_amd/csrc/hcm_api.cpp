@@ -722,8 +722,10 @@ extern "C" int hcm_batch_set_light(hcm_batch *b, uint32_t light_mode) {
 			HCM_HIP(dmalloc(&st->d_packed, (size_t)st->n_slots * kLightPlanes * 256));
 			HCM_HIP(dmalloc(&st->d_levels, st->n_slots));
 		}
-		if (!b->d_lctr)
+		if (!b->d_lctr) { // zero once: each light run's last workgroup clears the queues again
 			HCM_HIP(dmalloc(&b->d_lctr, 1));
+			HCM_HIP(hipMemset(b->d_lctr, 0, sizeof(Counters)));
+		}
 		if (!b->d_halo_light)
 			HCM_HIP(dmalloc(&b->d_halo_light, (size_t)b->n_jobs * kLightJobBytes));
 	}
@@ -808,10 +810,9 @@ static LightParams light_params(hcm_batch *b) {
 
 // Light rebuild of every job of the batch: pack the read slots, then the persistent light kernel.
 static hipError_t enqueue_light(hcm_batch *b, hipStream_t st) {
-	const LightParams lp = light_params(b);
-	hipError_t e = launch_reset(b->d_lctr, st);
-	if (e == hipSuccess)
-		e = launch_light_prep(lp, st);
+	LightParams lp = light_params(b);
+	lp.self_reset = 1;
+	hipError_t e = launch_light_prep(lp, st);
 	if (e == hipSuccess) {
 		const int per_cu = std::max(1, std::min(3, (160 * 1024) / std::max(1, light_lds_bytes())));
 		e = launch_light(lp, per_cu * b->ctx->n_cu, st);

@@ -102,6 +102,7 @@ struct LightParams {
 	const uint32_t *nbr;       // 27 slots per job
 	Counters *ctr;             // work queues of the persistent light kernel
 	uint8_t *halo_light;       // [n_jobs][kLightJobBytes]
+	uint32_t self_reset;       // the last workgroup to finish clears the work queues (no reset kernel)
 };
 
 // One voxel edit of a device chunk store: slot, chunk-local index, block, light. local >= 4096 sets the

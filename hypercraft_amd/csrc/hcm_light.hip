@@ -275,6 +275,15 @@ template <int NT> __global__ __launch_bounds__(NT, 3) void light_kernel(LightPar
 		light_job<NT>(p, s, job);
 		__syncthreads();
 	}
+	// as in the mesh kernel: the last workgroup to finish clears the queues for the next run
+	if (p.self_reset && threadIdx.x == 0) {
+		__threadfence();
+		if (atomicAdd(&p.ctr->done, 1u) == gridDim.x - 1) {
+			for (uint32_t q = 0; q < kQueues; ++q)
+				p.ctr->queue[q].v = 0;
+			p.ctr->done = 0;
+		}
+	}
 }
 
 hipError_t launch_light_prep(const LightParams &p, hipStream_t stream) {
