@@ -497,11 +497,10 @@ struct hcm_server {
 				lp.nbr = b.d_nbr;
 				lp.ctr = b.d_lctr;
 				lp.halo_light = b.d_halo;
-				HCM_CHECK(launch_reset(b.d_lctr, s_comp));
+				lp.self_reset = 1; // the slot's light queues start at zero; the run's last workgroup clears them
 				HCM_CHECK(launch_light_prep(lp, s_comp));
 				HCM_CHECK(launch_light(lp, std::max(1, std::min(info.n_cu * 3, (int)b.jobs.size())), s_comp));
 			}
-			HCM_CHECK(launch_reset(b.d_ctr, s_comp));
 			MeshParams p;
 			std::memset(&p, 0, sizeof(p));
 			p.reg = info.reg;
@@ -518,6 +517,7 @@ struct hcm_server {
 			p.rec_cap = rec_cap();
 			p.results = b.d_res;
 			p.halo_light = relight() ? b.d_halo : nullptr;
+			p.self_reset = 1; // the slot's counters start at zero; the run's last workgroup clears them
 			const int grid = (int)std::max<uint32_t>(1, std::min<uint32_t>(info.grid, (uint32_t)b.jobs.size()));
 			HCM_CHECK(launch_mesh(p, (int)info.threads, grid, s_comp));
 			HCM_CHECK(hipEventRecord(b.ev_k1, s_comp));
@@ -538,13 +538,13 @@ struct hcm_server {
 		if (!b.jobs.empty()) {
 			float ms = 0;
 			HCM_CHECK(hipEventElapsedTime(&ms, b.ev_k0, b.ev_k1));
-			const uint32_t n_recs = std::min(b.h_ctr->mesh_cursor, rec_cap());
+			const uint32_t n_recs = std::min(b.h_ctr->total_meshes, rec_cap());
 			if (n_recs > b.recs_copied) // more overflow meshes than usual: fetch the rest
 				HCM_CHECK(hipMemcpy(b.h_recs + b.recs_copied, b.d_recs + b.recs_copied,
 				                    (n_recs - b.recs_copied) * sizeof(hcm_mesh_rec), hipMemcpyDeviceToHost));
 			std::vector<uint2> hv;
 			std::vector<uint16_t> hi;
-			const uint64_t used = std::min<uint64_t>(b.h_ctr->pool_cursor, b.arena.count);
+			const uint64_t used = std::min<uint64_t>(b.h_ctr->total_quads, b.arena.count);
 			if (cfg.keep_history && used) {
 				hv.resize(used * 4);
 				hi.resize(used * 6);
@@ -771,6 +771,7 @@ static int server_init(hcm_server *s) {
 		HCM_CHECK(dalloc(&b.d_nbr, (size_t)c.max_batch * 27));
 		HCM_CHECK(dalloc(&b.d_edits, kMaxEdits));
 		HCM_CHECK(dalloc(&b.d_ctr, 1));
+		HCM_CHECK(hipMemset(b.d_ctr, 0, sizeof(Counters)));
 		HCM_CHECK(dalloc(&b.d_res, c.max_batch));
 		HCM_CHECK(dalloc(&b.d_recs, s->rec_cap()));
 		HCM_CHECK(halloc(&b.h_nbr, (size_t)c.max_batch * 27));
@@ -782,6 +783,7 @@ static int server_init(hcm_server *s) {
 			HCM_CHECK(dalloc(&b.d_rslots, (size_t)c.max_batch * 27));
 			HCM_CHECK(halloc(&b.h_rslots, (size_t)c.max_batch * 27));
 			HCM_CHECK(dalloc(&b.d_lctr, 1));
+			HCM_CHECK(hipMemset(b.d_lctr, 0, sizeof(Counters)));
 			HCM_CHECK(dalloc(&b.d_halo, (size_t)c.max_batch * kLightJobBytes));
 		}
 		HCM_CHECK(halloc(&b.h_res, c.max_batch));
