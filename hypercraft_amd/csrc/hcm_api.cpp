@@ -1,9 +1,9 @@
 // hcm_api.cpp -- host side of the C ABI declared in include/hcm.h.
 //
-// Owns device memory (registry tables, quad pools, chunk stores, job tables), the HIP stream each
-// context works on, and the HIP graph that replays the batched pipeline (counter resets, the light
-// rebuild when asked for, the persistent meshing kernel). Kernels live in hcm_kernels.hip and
-// hcm_light.hip.
+// Owns device memory (registry tables, quad pools, chunk stores, job tables, depth hierarchies), the HIP
+// stream each context works on, and the HIP graph that replays the batched pipeline (the light rebuild
+// when asked for, then the persistent meshing kernel; both clear their own counters). Kernels live in
+// hcm_kernels.hip, hcm_light.hip and hcm_draw.hip.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
