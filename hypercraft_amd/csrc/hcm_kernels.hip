@@ -940,11 +940,9 @@ template <int NT, typename Fn> __device__ __forceinline__ void describe_row(cons
 // ------------------------------------------------------------------------------------------------
 // one chunk
 // ------------------------------------------------------------------------------------------------
-template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p, Smem<NT> &s, uint32_t job) {
-	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-	constexpr int CPT = HCM_CHUNK_VOXELS / NT; // custom cells per thread
-	if (tid < 27)
-		s.slots[tid] = p.nbr[27 * (size_t)job + tid];
+// Per-chunk LDS state a job starts from (the slot list is loaded with the job).
+template <int NT> __device__ __forceinline__ void chunk_init(Smem<NT> &s) {
+	const int tid = threadIdx.x;
 	for (int i = tid; i < 128; i += NT)
 		s.custom_bits[i] = 0;
 	if (tid < 2 * kMaxMeshPerPass * 6) {
@@ -953,7 +951,12 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 	}
 	if (tid == 0)
 		s.n_events = 0, s.flags = 0;
-	__syncthreads();
+}
+
+// One job; s.slots and the state of chunk_init are in place (a barrier after them).
+template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p, Smem<NT> &s, uint32_t job) {
+	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+	constexpr int CPT = HCM_CHUNK_VOXELS / NT; // custom cells per thread
 
 	// 1. halo staging: one thread per x-row (hy, hz) of 18 voxels; the lookups of a row are issued as
 	// batches (block id -> texture row in LDS, the rare meta-dependent ids from the full table, then the
@@ -1398,8 +1401,11 @@ template <int NT> __global__ __launch_bounds__(NT, NT >= 512 ? 2 : 3) void mesh_
 	for (int i = threadIdx.x; i < 256; i += NT)
 		s.pid_id[i] = p.reg.pid_of[65536 + i];
 	if (!p.persistent) {
-		__syncthreads();
 		const uint32_t job = p.job_base + blockIdx.x;
+		if (job < p.n_jobs && threadIdx.x < 27)
+			s.slots[threadIdx.x] = p.nbr[27 * (size_t)job + threadIdx.x];
+		chunk_init(s);
+		__syncthreads();
 		if (job < p.n_jobs)
 			mesh_chunk(p, s, job);
 		return;
@@ -1407,22 +1413,30 @@ template <int NT> __global__ __launch_bounds__(NT, NT >= 512 ? 2 : 3) void mesh_
 	// part q of the job list is [q * n / kQueues, (q + 1) * n / kQueues); a part is empty once its
 	// counter reaches its length, and the workgroup moves on to the next part
 	const uint32_t home = blockIdx.x % kQueues;
-	uint32_t tried = 0;
+	uint32_t tried = 0; // lane 0 of wavefront 0 only
+	const int lane = threadIdx.x & 63;
 	for (;;) {
-		if (threadIdx.x == 0) {
+		// wavefront 0 claims the next job and loads its slot list while the others set up its LDS state
+		if (threadIdx.x < 64) {
 			uint32_t job = ~0u;
-			while (job == ~0u && tried < kQueues) {
-				const uint32_t q = (home + tried) % kQueues;
-				const uint32_t lo = uint32_t(uint64_t(q) * p.n_jobs / kQueues);
-				const uint32_t len = uint32_t(uint64_t(q + 1) * p.n_jobs / kQueues) - lo;
-				const uint32_t k = atomicAdd(&p.ctr->queue[q].v, 1u);
-				if (k < len)
-					job = lo + k;
-				else
-					++tried;
+			if (lane == 0) {
+				while (job == ~0u && tried < kQueues) {
+					const uint32_t q = (home + tried) % kQueues;
+					const uint32_t lo = uint32_t(uint64_t(q) * p.n_jobs / kQueues);
+					const uint32_t len = uint32_t(uint64_t(q + 1) * p.n_jobs / kQueues) - lo;
+					const uint32_t k = atomicAdd(&p.ctr->queue[q].v, 1u);
+					if (k < len)
+						job = lo + k;
+					else
+						++tried;
+				}
+				s.job = job;
 			}
-			s.job = job;
+			job = (uint32_t)__shfl((int)job, 0);
+			if (job < p.n_jobs && lane < 27)
+				s.slots[lane] = p.nbr[27 * (size_t)job + lane];
 		}
+		chunk_init(s);
 		__syncthreads();
 		const uint32_t job = s.job;
 		if (job >= p.n_jobs)
