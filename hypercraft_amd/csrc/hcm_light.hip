@@ -145,20 +145,25 @@ __device__ __forceinline__ void propagate(const LightParams &p, LightSmem &s, bo
 	}
 }
 
+// Wavefront 0: the slot list of `job` and the torchlight levels present in its 27 chunks, into LDS.
+__device__ __forceinline__ void light_slots(const LightParams &p, LightSmem &s, uint32_t job) {
+	const int lane = threadIdx.x & 63;
+	uint32_t slot = 0, lv = 0;
+	if (lane < 27) {
+		slot = p.nbr[27 * (size_t)job + lane];
+		s.slots[lane] = slot;
+		lv = p.levels[slot];
+	}
+#pragma unroll
+	for (int d = 1; d < 64; d <<= 1)
+		lv |= __shfl_xor(lv, d);
+	if (lane == 0)
+		s.levels = lv & ~1u;
+}
+
+// One job; s.slots and s.levels are in place (a barrier after them).
 template <int NT> __device__ __forceinline__ void light_job(const LightParams &p, LightSmem &s, uint32_t job) {
 	const int tid = threadIdx.x;
-	if (tid < 27)
-		s.slots[tid] = p.nbr[27 * (size_t)job + tid];
-	__syncthreads();
-	if (tid < 64) {
-		uint32_t lv = tid < 27 ? p.levels[s.slots[tid]] : 0u;
-#pragma unroll
-		for (int d = 1; d < 64; d <<= 1)
-			lv |= __shfl_xor(lv, d);
-		if (tid == 0)
-			s.levels = lv & ~1u;
-	}
-	__syncthreads();
 	const bool torch = s.levels != 0, torch15 = (s.levels >> 15) & 1u;
 	for (int r = tid; r < kWRows; r += NT) {
 		const int wy = r / kW, wz = r % kW;
@@ -252,21 +257,26 @@ template <int NT> __global__ __launch_bounds__(NT, 3) void light_kernel(LightPar
 	// part q of the job list is [q * n / kQueues, (q + 1) * n / kQueues): a workgroup takes jobs from
 	// its own XCD's part first, then from the others
 	const uint32_t home = blockIdx.x % kQueues;
-	uint32_t tried = 0;
+	uint32_t tried = 0; // lane 0 of wavefront 0 only
 	for (;;) {
-		if (threadIdx.x == 0) {
+		if (threadIdx.x < 64) { // wavefront 0 claims the next job and loads its slot list and levels
 			uint32_t job = ~0u;
-			while (job == ~0u && tried < kQueues) {
-				const uint32_t q = (home + tried) % kQueues;
-				const uint32_t lo = uint32_t(uint64_t(q) * p.n_jobs / kQueues);
-				const uint32_t len = uint32_t(uint64_t(q + 1) * p.n_jobs / kQueues) - lo;
-				const uint32_t k = atomicAdd(&p.ctr->queue[q].v, 1u);
-				if (k < len)
-					job = lo + k;
-				else
-					++tried;
+			if (threadIdx.x == 0) {
+				while (job == ~0u && tried < kQueues) {
+					const uint32_t q = (home + tried) % kQueues;
+					const uint32_t lo = uint32_t(uint64_t(q) * p.n_jobs / kQueues);
+					const uint32_t len = uint32_t(uint64_t(q + 1) * p.n_jobs / kQueues) - lo;
+					const uint32_t k = atomicAdd(&p.ctr->queue[q].v, 1u);
+					if (k < len)
+						job = lo + k;
+					else
+						++tried;
+				}
+				s.job = job;
 			}
-			s.job = job;
+			job = (uint32_t)__shfl((int)job, 0);
+			if (job < p.n_jobs)
+				light_slots(p, s, job);
 		}
 		__syncthreads();
 		const uint32_t job = s.job;
