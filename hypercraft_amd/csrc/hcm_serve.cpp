@@ -16,8 +16,11 @@
 // has applied it and every chunk it made stale holds a mesh from a batch at or after its epoch.
 // Remesh rules follow the reference: a chunk is meshed only when all 26 neighbours are present
 // (client/src/ChunkMeshTask.cpp:14-39), and an edit invalidates every chunk whose 18^3 halo holds the
-// voxel -- the edited chunk plus the face/edge/corner neighbours the voxel touches.
+// voxel -- the edited chunk plus the face/edge/corner neighbours the voxel touches. Remeshes an edit
+// causes go ahead of those of loads when batches are formed, as the reference pushes them at high
+// priority (client/src/ChunkSetBlockTask.cpp:119, served first by client/src/ChunkTaskPool.cpp:97-105).
 #include <hip/hip_runtime.h>
+#include <sys/prctl.h>
 
 #include <algorithm>
 #include <atomic>
@@ -33,6 +36,7 @@
 #include <vector>
 
 #include "hcm_internal.h"
+#include "hcm_range_alloc.h"
 
 using namespace hcm;
 
@@ -64,65 +68,10 @@ constexpr uint32_t kMaxUploads = 1024;  // chunk uploads staged per batch
 constexpr uint32_t kMaxEdits = 65536;   // voxel edits applied per batch
 constexpr uint64_t kMinArena = 1 << 16; // quads; more than any one chunk's meshes
 
-struct Range {
-	uint64_t first = 0, count = 0;
-};
-
-// First-fit allocator of quad ranges in [0, capacity).
-class RangeAllocator {
-public:
-	explicit RangeAllocator(uint64_t cap) : m_cap(cap), m_free_total(cap) {
-		if (cap)
-			m_free[0] = cap;
-	}
-	bool alloc(uint64_t n, uint64_t *first) {
-		for (auto it = m_free.begin(); it != m_free.end(); ++it)
-			if (it->second >= n) {
-				*first = it->first;
-				const uint64_t rest = it->second - n, at = it->first + n;
-				m_free.erase(it);
-				if (rest)
-					m_free[at] = rest;
-				m_free_total -= n;
-				return true;
-			}
-		return false;
-	}
-	void release(uint64_t first, uint64_t n) {
-		if (!n)
-			return;
-		m_free_total += n;
-		auto it = m_free.emplace(first, n).first;
-		auto nx = std::next(it);
-		if (nx != m_free.end() && it->first + it->second == nx->first) {
-			it->second += nx->second;
-			m_free.erase(nx);
-		}
-		if (it != m_free.begin()) {
-			auto pv = std::prev(it);
-			if (pv->first + pv->second == it->first) {
-				pv->second += it->second;
-				m_free.erase(it);
-			}
-		}
-	}
-	uint64_t largest() const {
-		uint64_t m = 0;
-		for (auto &f : m_free)
-			m = std::max(m, f.second);
-		return m;
-	}
-	uint64_t free_total() const { return m_free_total; }
-	uint64_t capacity() const { return m_cap; }
-
-private:
-	uint64_t m_cap, m_free_total;
-	std::map<uint64_t, uint64_t> m_free; // first -> count
-};
-
 struct Chunk {
 	uint32_t slot = 0;
 	bool stale = false;             // mesh out of date and not in a launched batch
+	bool urgent = false;            // stale because of an edit: queued with high priority
 	bool has_mesh = false;
 	uint64_t meshed_epoch = 0;      // epoch of the batch that produced the current meshes
 	Range mesh;                     // their pool range
@@ -181,7 +130,8 @@ struct BatchSlot { // device and pinned buffers of one in-flight batch
 	bool arena_was_largest = false; // the arena took the largest free range there was
 	uint64_t id = 0, epoch = 0;     // epoch: every request up to it is applied on the device
 	uint32_t recs_copied = 0;       // mesh records copied back with the results (the usual count)
-	std::vector<uint64_t> jobs;
+	std::vector<uint64_t> jobs;     // the first n_urgent were taken from the high-priority queue
+	uint32_t n_urgent = 0;
 	Range arena;
 };
 
@@ -221,7 +171,9 @@ struct hcm_server {
 	// scheduler state: changed by the scheduler thread under world_mu
 	std::mutex world_mu;
 	std::unordered_map<uint64_t, Chunk> world;
-	std::deque<uint64_t> stale_fifo;
+	std::deque<uint64_t> stale_fifo; // stale chunks by time (loads, sunlight changes)
+	std::deque<uint64_t> stale_hi;   // stale chunks an edit remeshes: taken first
+	bool pool_blocked = false;       // an arena did not fit while batches ran: wait for one to finish
 	std::vector<std::unique_ptr<Request>> waiting;
 	std::deque<Upload> uploads;
 	std::map<uint64_t, PendingEdit> edits; // (slot << 13 | local) -> last edit of that voxel or height
@@ -258,16 +210,26 @@ struct hcm_server {
 		}
 		return true;
 	}
-	void mark_stale(int32_t cx, int32_t cy, int32_t cz, std::vector<uint64_t> &targets) {
+	// A chunk made stale is queued once per priority: a chunk already queued by time moves ahead when an
+	// edit marks it (ChunkMeshTask.inl:40 keeps the high priority flag until the task runs); the `stale`
+	// flag makes the later of two queue entries a no-op.
+	void queue_stale(uint64_t key, Chunk &c, bool urgent) {
+		if (!c.stale) {
+			c.stale = true;
+			c.urgent = urgent;
+			(urgent ? stale_hi : stale_fifo).push_back(key);
+		} else if (urgent && !c.urgent) {
+			c.urgent = true;
+			stale_hi.push_back(key);
+		}
+	}
+	void mark_stale(int32_t cx, int32_t cy, int32_t cz, std::vector<uint64_t> &targets, bool urgent) {
 		const int32_t p[3] = {cx, cy, cz};
 		const uint64_t key = chunk_key(cx, cy, cz);
 		auto it = world.find(key);
 		if (it == world.end() || !meshable(p))
 			return;
-		if (!it->second.stale) {
-			it->second.stale = true;
-			stale_fifo.push_back(key);
-		}
+		queue_stale(key, it->second, urgent);
 		targets.push_back(key);
 	}
 
@@ -295,13 +257,20 @@ struct hcm_server {
 			for (int s = 0; s < 27; ++s) {
 				int d[3];
 				slot_offset(s, d);
-				mark_stale(r->pos[0] + d[0], r->pos[1] + d[1], r->pos[2] + d[2], r->targets);
+				mark_stale(r->pos[0] + d[0], r->pos[1] + d[1], r->pos[2] + d[2], r->targets, false);
 			}
 		} else {
 			const int32_t cx = floor_div16(r->pos[0]), cy = floor_div16(r->pos[1]), cz = floor_div16(r->pos[2]);
 			const int32_t lx = r->pos[0] - 16 * cx, ly = r->pos[1] - 16 * cy, lz = r->pos[2] - 16 * cz;
 			Chunk &c = world.at(chunk_key(cx, cy, cz));
 			const uint32_t local = uint32_t(lx) + 16u * (uint32_t(lz) + 16u * uint32_t(ly));
+			if (relight() && c.blocks[local] == r->block) {
+				// the block does not change: nothing to apply or remesh (ChunkSetBlockTask.cpp:89); the light
+				// byte of an edit is not used when light is rebuilt. (A light-store server keeps no host copy
+				// of the voxels and applies every edit.)
+				waiting.push_back(std::move(r));
+				return;
+			}
 			auto pend = [&](const VoxelEdit &ve) {
 				const uint64_t vkey = (uint64_t(ve.slot) << 13) | ve.local;
 				auto e = edits.find(vkey);
@@ -332,7 +301,7 @@ struct hcm_server {
 						for (int s = 0; s < 27; ++s) {
 							int d[3];
 							slot_offset(s, d);
-							mark_stale(cx + d[0], cy + d[1], cz + d[2], r->targets);
+							mark_stale(cx + d[0], cy + d[1], cz + d[2], r->targets, false); // ChunkSetSunlightTask.cpp:84
 						}
 					}
 				}
@@ -345,7 +314,7 @@ struct hcm_server {
 					int d[3];
 					slot_offset(s, d);
 					if (dist(lx - 16 * d[0]) + dist(ly - 16 * d[1]) + dist(lz - 16 * d[2]) < 15)
-						mark_stale(cx + d[0], cy + d[1], cz + d[2], r->targets);
+						mark_stale(cx + d[0], cy + d[1], cz + d[2], r->targets, true);
 				}
 			} else {
 				// a voxel on a chunk border lies in the halo of the neighbours across that border
@@ -355,7 +324,7 @@ struct hcm_server {
 				for (int a = 0; a < (ox[1] ? 2 : 1); ++a)
 					for (int b = 0; b < (oy[1] ? 2 : 1); ++b)
 						for (int d = 0; d < (oz[1] ? 2 : 1); ++d)
-							mark_stale(cx + ox[a], cy + oy[b], cz + oz[d], r->targets);
+							mark_stale(cx + ox[a], cy + oy[b], cz + oz[d], r->targets, true);
 			}
 			std::sort(r->targets.begin(), r->targets.end()); // a chunk marked by two rules counts once
 			r->targets.erase(std::unique(r->targets.begin(), r->targets.end()), r->targets.end());
@@ -369,7 +338,9 @@ struct hcm_server {
 				return true;
 		return false;
 	}
-	bool work_pending() const { return !stale_fifo.empty() || !uploads.empty() || !edits.empty() || any_busy(); }
+	bool work_pending() const {
+		return !stale_hi.empty() || !stale_fifo.empty() || !uploads.empty() || !edits.empty() || any_busy();
+	}
 
 	// Returns 0 when launched, 1 when there is nothing to launch now, or an error.
 	int launch_batch(BatchSlot &b) {
@@ -377,17 +348,24 @@ struct hcm_server {
 		const bool up_rest = uploads.size() > n_up;
 		const uint32_t n_ed = up_rest ? 0u : (uint32_t)std::min<size_t>(edits.size(), kMaxEdits);
 		const bool ed_rest = edits.size() > n_ed;
-		// meshing only once every pending change is in this batch
+		// meshing only once every pending change is in this batch, and not while an arena that did not
+		// fit waits for a running batch to return memory; edit remeshes first
 		b.jobs.clear();
-		if (!up_rest && !ed_rest)
-			while (!stale_fifo.empty() && b.jobs.size() < cfg.max_batch) {
-				const uint64_t key = stale_fifo.front();
-				stale_fifo.pop_front();
-				auto it = world.find(key);
-				if (it != world.end() && it->second.stale) { // a key queued twice is taken once
-					it->second.stale = false;
-					b.jobs.push_back(key);
+		b.n_urgent = 0;
+		if (!up_rest && !ed_rest && !pool_blocked)
+			for (std::deque<uint64_t> *q : {&stale_hi, &stale_fifo}) {
+				while (!q->empty() && b.jobs.size() < cfg.max_batch) {
+					const uint64_t key = q->front();
+					q->pop_front();
+					auto it = world.find(key);
+					if (it != world.end() && it->second.stale) { // a key queued twice is taken once
+						it->second.stale = false;
+						it->second.urgent = false;
+						b.jobs.push_back(key);
+					}
 				}
+				if (q == &stale_hi)
+					b.n_urgent = (uint32_t)b.jobs.size();
 			}
 		// arena admission by vertex-pool memory
 		b.arena = Range{};
@@ -405,12 +383,16 @@ struct hcm_server {
 				ok = pool->alloc(got, &first);
 			}
 			b.arena_was_largest = ok && got >= largest;
-			if (!ok) { // wait for running batches to return memory
-				for (auto it = b.jobs.rbegin(); it != b.jobs.rend(); ++it) {
-					world.at(*it).stale = true;
-					stale_fifo.push_front(*it);
+			if (!ok) { // wait for running batches to return memory, the jobs back at the front of their queues
+				for (size_t j = b.jobs.size(); j-- > 0;) {
+					Chunk &c = world.at(b.jobs[j]);
+					c.stale = true;
+					c.urgent = j < b.n_urgent;
+					(c.urgent ? stale_hi : stale_fifo).push_front(b.jobs[j]);
 				}
 				b.jobs.clear();
+				b.n_urgent = 0;
+				pool_blocked = true;
 			} else
 				b.arena = Range{first, got};
 		}
@@ -559,10 +541,7 @@ struct hcm_server {
 				const hcm_job_result &res = b.h_res[j];
 				if (res.status != HCM_OK) { // arena or record space ran out: mesh it again later
 					++retries;
-					if (!c.stale) {
-						c.stale = true;
-						stale_fifo.push_back(key);
-					}
+					queue_stale(key, c, j < b.n_urgent);
 					continue;
 				}
 				uint64_t lo = 0;
@@ -622,10 +601,15 @@ struct hcm_server {
 		}
 		applied_done = std::max(applied_done, b.epoch);
 		b.busy = false;
+		pool_blocked = false; // its arena's unused part and replaced meshes went back to the pool
 		return HCM_OK;
 	}
 
 	void complete_ready() {
+		// with no data change pending or in flight, the device holds every request's data (a request that
+		// changed nothing, such as an edit that sets a voxel to its block, then completes at once)
+		if (uploads.empty() && edits.empty() && !any_busy())
+			applied_done = epoch;
 		const auto now = Clock::now();
 		std::vector<hcm_completion> out;
 		for (auto &r : waiting) {
@@ -659,7 +643,10 @@ struct hcm_server {
 	}
 
 	void run() {
+		// short naps while batches run (below) should last microseconds, not the default 50 us timer slack
+		prctl(PR_SET_TIMERSLACK, 1000UL, 0, 0, 0);
 		for (;;) {
+			bool progress = false;
 			std::vector<std::unique_ptr<Request>> got;
 			{
 				std::unique_lock<std::mutex> lk(in_mu);
@@ -677,7 +664,8 @@ struct hcm_server {
 					incoming.pop_front();
 				}
 			}
-			std::lock_guard<std::mutex> wl(world_mu);
+			std::unique_lock<std::mutex> wl(world_mu);
+			progress = !got.empty();
 			for (auto &r : got)
 				apply(std::move(r));
 			for (;;) { // retire finished batches in launch order
@@ -693,6 +681,7 @@ struct hcm_server {
 				const int rc = q == hipSuccess ? finish_batch(*oldest) : HCM_ERR_HIP;
 				if (rc != HCM_OK)
 					return fail(rc);
+				progress = true;
 			}
 			complete_ready();
 			for (;;) {
@@ -711,7 +700,14 @@ struct hcm_server {
 					return fail(rc);
 				if (rc > 0)
 					break;
+				progress = true;
 			}
+			wl.unlock();
+			// nothing arrived, finished or launched in this pass: batches are running (an idle server waits
+			// on in_cv above). Nap briefly instead of polling their events and the locks in a tight loop; a
+			// request submitted meanwhile is taken up after at most one nap.
+			if (!progress)
+				std::this_thread::sleep_for(std::chrono::microseconds(10));
 		}
 	}
 };

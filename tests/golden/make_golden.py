@@ -11,9 +11,12 @@ compiled from its own sources). The fixtures are data only:
   light.npz     the reference's light levels (Block::GetLightLevel by Block::m_data) and, for each
                 light set, per job the SHA-256 of the rebuilt light over the mesher's halo and the
                 mesh digest over that light (the mesh task, client/src/ChunkMeshTask.cpp:41-127).
+  trace.npz     for every meshable chunk of the spiral serving trace, the digest of the reference's
+                meshes over the trace's final world (every load and edit applied).
 
     python tests/golden/make_golden.py [set ...]   (no names: every set)
     python tests/golden/make_golden.py light
+    python tests/golden/make_golden.py trace
 """
 import os
 import sys
@@ -126,8 +129,55 @@ def main(only=None):
     np.savez_compressed(path, **out)
 
 
+def trace_final_world(tr):
+    """The spiral serving trace's world after every request, in arrival order: each chunk's generated
+    blocks and light bytes, then every edit's voxel (the store-light server applies an edit's block and
+    light byte, ChunkSetBlockTask.cpp:100-119 with light an input). Returns (blocks, light) by chunk index."""
+    coords = tr["coords"]
+    blocks, light = tr["blocks"].copy(), tr["light"].copy()
+    index = {tuple(c): i for i, c in enumerate(coords.tolist())}
+    for _, kind, payload in tr["events"]:
+        if kind == "edit":
+            x, y, z, blk, lt = payload
+            i = index[(x // 16, y // 16, z // 16)]
+            v = (x % 16) + 16 * ((z % 16) + 16 * (y % 16))
+            blocks[i, v], light[i, v] = blk, lt
+    return blocks, light
+
+
+def trace_meshable(tr):
+    """The trace's meshable chunks (all 26 neighbours loaded), by chunk index, and their 27 chunk indices."""
+    coords = tr["coords"]
+    index = {tuple(c): i for i, c in enumerate(coords.tolist())}
+    nbr = []
+    for c in coords.tolist():
+        row = [index.get((c[0] + d[0], c[1] + d[1], c[2] + d[2])) for d in workloads.SLOT_OFFSETS.tolist()]
+        nbr.append(row if None not in row else None)
+    pick = np.array([i for i, r in enumerate(nbr) if r is not None])
+    return pick, np.array([nbr[i] for i in pick], np.uint32)
+
+
+def main_trace():
+    """Writes trace.npz: for every meshable chunk of the spiral trace, the reference's mesh digest over the
+    final world (every load and edit applied), and the chunks' positions."""
+    ref = cpu_oracle.Reference()
+    tr = workloads.spiral_trace()
+    blocks, light = trace_final_world(tr)
+    pick, nbr = trace_meshable(tr)
+    dig = np.zeros((pick.size, 32), np.uint8)
+    for j in range(pick.size):
+        dig[j] = np.frombuffer(meshdigest.digest(ref.mesh(*workloads.slice_jobs(blocks, light, nbr, j))), np.uint8)
+    n_edits = sum(1 for e in tr["events"] if e[1] == "edit")
+    np.savez_compressed(os.path.join(HERE, "trace.npz"), positions=tr["coords"][pick].astype(np.int32),
+                        digest=dig, input=np.frombuffer(meshdigest.input_digest(blocks, light, nbr).encode(), np.uint8),
+                        counts=np.array([tr["coords"].shape[0], n_edits], np.int64))
+    print(f"trace: {pick.size} meshable chunks of {tr['coords'].shape[0]}, {n_edits} edits")
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["light"]:
         main_light()
+    elif sys.argv[1:] == ["trace"]:
+        main_trace()
     else:
         main(sys.argv[1:])

@@ -246,10 +246,14 @@ class LitWorld(World):
 
     def edit(self, x, y, z, block, light):
         """Sets the block, then floods sunlight down the edited column when the chunk above is loaded
-        (ChunkFloodSunlightTask.cpp:38-54). Returns whether the column's sunlight height changed."""
+        (ChunkFloodSunlightTask.cpp:38-54). Returns whether the column's sunlight height changed, or None
+        when the voxel already holds the block: such an edit changes nothing (ChunkSetBlockTask.cpp:89)."""
         c = (x // 16, y // 16, z // 16)
         blocks, heights = self.chunks[c]
-        blocks[(x % 16) + 16 * ((z % 16) + 16 * (y % 16))] = block
+        i = (x % 16) + 16 * ((z % 16) + 16 * (y % 16))
+        if blocks[i] == block:
+            return None
+        blocks[i] = block
         up = (c[0], c[1] + 1, c[2])
         if up not in self.chunks:
             return False
@@ -307,23 +311,30 @@ def test_server_relight_matches_mesh_task_alone(ctx, oracle, max_batch, max_infl
             log.append(("load", pos, b, h))
         say("serve relight: 120 edits")
         palette = np.array([0, 0, 1, 3, 9, 0x18, 0x10, 0x0E], np.uint16)  # air, stone, dirt, glass, glowstone, leaves, water
-        flooded = set()
+        flooded, noop = set(), set()
         for k, (x, y, z, _, _) in enumerate(_edits(rng, 120)):
             blk = int(palette[rng.integers(len(palette))])
+            if k % 10 == 9:  # every tenth edit sets a voxel to the block it holds
+                c = (x // 16, y // 16, z // 16)
+                blk = int(world.chunks[c][0][(x % 16) + 16 * ((z % 16) + 16 * (y % 16))])
             srv.edit(x, y, z, blk, 0, tag=len(log))
-            if world.edit(x, y, z, blk, 0):
+            changed = world.edit(x, y, z, blk, 0)
+            if changed:
                 flooded.add(len(log))
+            elif changed is None:
+                noop.add(len(log))
             log.append(("edit", x, y, z, blk, 0))
             if k % 40 == 39:
                 srv.drain()
         assert flooded, "some edits move a sunlight height"
+        assert len(noop) >= 12, "some edits change nothing"
         srv.drain()
         done = srv.poll()
         check_completions(done, len(log), log)
         for c in done:
             if c["kind"] == hcm.REQ_EDIT:
                 _, x, y, z = log[c["tag"]][:4]
-                want = light_chunks(world, x, y, z)
+                want = set() if c["tag"] in noop else light_chunks(world, x, y, z)
                 if c["tag"] in flooded:  # a moved sunlight height remeshes the chunk and its 26 neighbours
                     want |= {n for n in (tuple(int(v) for v in np.add((x // 16, y // 16, z // 16), d))
                                          for d in workloads.SLOT_OFFSETS) if world.meshable(n)}
@@ -333,6 +344,71 @@ def test_server_relight_matches_mesh_task_alone(ctx, oracle, max_batch, max_infl
         check_final(srv, oracle, world)
         with pytest.raises(hcm.HcmError):
             srv.load(0, 0, 0, *terrain((0, 0, 0)))  # light bytes are not what this server takes
+    finally:
+        srv.close()
+
+
+def test_server_edit_remesh_goes_first(ctx, oracle):
+    """Remeshes an edit causes are taken before those of earlier loads (ChunkSetBlockTask.cpp:119 pushes
+    them at high priority): behind a backlog of 108 reloads (16 chunks to remesh, one per batch), the first
+    mesh made at or after the edit's epoch is the edited chunk's, which the reloads queued last."""
+    dims = (6, 3, 6)
+    chunks = [(x, y, z) for y in range(dims[1]) for z in range(dims[2]) for x in range(dims[0])]
+    srv = hcm.Server(ctx, max_chunks=128, max_batch=1, max_inflight=1, pool_quads=1 << 22, keep_history=True)
+    try:
+        world, log = World(), []
+        load_all(srv, world, log, chunks)
+        srv.drain()
+        say("serve: 108 reloads, then an edit in the chunk they remesh last")
+        for pos in chunks:
+            b, l = hcm.gen_cave_chunk(SEED, *pos)
+            srv.load(*pos, b, l, tag=len(log))
+            world.load(pos, b, l)
+            log.append(("load", pos, b, l))
+        x, y, z = 16 * 4 + 7, 16 * 1 + 8, 16 * 4 + 9  # inside chunk (4, 1, 4), away from its borders
+        srv.edit(x, y, z, 9, 0x3A, tag=len(log))
+        world.edit(x, y, z, 9, 0x3A)
+        log.append(("edit", x, y, z, 9, 0x3A))
+        edit_epoch = len(log)
+        srv.drain()
+        done = srv.poll()
+        check_completions(done, len(log), log)
+        after = [h for h in srv.history() if h[1] >= edit_epoch]
+        assert after and tuple(after[0][0]) == (4, 1, 4), [tuple(h[0]) for h in after[:4]]
+        assert sum(1 for h in after if tuple(h[0]) == (4, 1, 4)) == 1
+        edit_done = next(c for c in done if c["tag"] == edit_epoch - 1)
+        assert edit_done["remeshed"] == 1
+        check_history(srv, oracle, log)
+        check_final(srv, oracle, world)
+    finally:
+        srv.close()
+
+
+def test_server_spiral_trace_final_meshes(ctx):
+    """The whole serving trace of the benchmark (spiral loads of 28,688 terrain chunks, radius 24 x 16
+    chunks, and 2,911 single-voxel edits), submitted as fast as the server takes it: every request
+    completes, and each of the 22,470 meshable chunks then holds the mesh the reference mesher makes of
+    it on the trace's final world (tests/golden/trace.npz, written by make_golden.py trace)."""
+    gold = np.load(os.path.join(GOLDEN, "trace.npz"))
+    tr = workloads.spiral_trace()
+    coords, blocks, light, events = tr["coords"], tr["blocks"], tr["light"], tr["events"]
+    assert gold["counts"].tolist() == [coords.shape[0], sum(1 for e in events if e[1] == "edit")]
+    srv = hcm.Server(ctx, max_chunks=coords.shape[0], max_batch=4096, max_inflight=2, pool_quads=1 << 24)
+    try:
+        say(f"serve: spiral trace, {len(events)} requests")
+        for tag, (_, kind, payload) in enumerate(events):
+            if kind == "load":
+                c = coords[payload]
+                srv.load(int(c[0]), int(c[1]), int(c[2]), blocks[payload], light[payload], tag)
+            else:
+                srv.edit(*payload, tag)
+        srv.drain()
+        done = srv.poll()
+        assert sorted(c["tag"] for c in done) == list(range(len(events)))
+        say(f"serve: comparing {gold['positions'].shape[0]} final meshes with the reference's")
+        bad = [tuple(p) for p, d in zip(gold["positions"].tolist(), gold["digest"])
+               if meshdigest.digest(srv.chunk_mesh(*p)) != d.tobytes()]
+        assert not bad, f"{len(bad)} chunks differ, first {bad[:5]}"
     finally:
         srv.close()
 
