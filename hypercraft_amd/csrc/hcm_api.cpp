@@ -73,14 +73,34 @@ int build_tables(const hcm_registry *r, DeviceTables *t) {
 		code_tex.push_back(tex);
 		return (uint32_t)code_tex.size() - 1;
 	};
+	// BlockTexture::Show (texture/include/texture/BlockTexture.hpp:68-76) on full texture values; texture
+	// id 0 counts as transparent (block_texture_transparency.inl, index 0)
+	auto show = [&](uint16_t self, uint16_t other) -> bool {
+		const bool s_empty = (self & 0x1fffu) == 0, o_empty = (other & 0x1fffu) == 0;
+		const bool s_t = s_empty || (tex_flags(self) & HCM_TEX_TRANSPARENT);
+		const bool o_t = o_empty || (tex_flags(other) & HCM_TEX_TRANSPARENT);
+		const bool s_l = !s_empty && (tex_flags(self) & HCM_TEX_LIQUID);
+		const bool o_l = !o_empty && (tex_flags(other) & HCM_TEX_LIQUID);
+		if (s_empty || self == other || (!s_t && !o_t) || (s_l && !o_empty))
+			return false;
+		return !s_t || o_t || o_l;
+	};
 	auto add_row = [&](const uint16_t tex[6], uint32_t dp, uint32_t ip, uint32_t mesh, uint32_t flags) {
 		uint32_t tc[6], cls = 0;
 		for (int f = 0; f < 6; ++f) {
 			tc[f] = code_of(tex[f]);
 			cls |= tex_class(tex[f]) << (2 * f);
 		}
+		// kRowSelfShow: two neighbouring blocks of this row show a face to each other along some axis
+		// (their + and - textures differ and one of them shows against the other), so a halo made of
+		// this one block value still has faces
+		uint32_t self_show = 0;
+		for (int a = 0; a < 3; ++a)
+			if (show(tex[2 * a], tex[2 * a + 1]) || show(tex[2 * a + 1], tex[2 * a]))
+				self_show = kRowSelfShow;
 		const uint32_t pid = (uint32_t)t->rows.size();
-		t->rows.push_back(make_uint2(tc[0] | tc[1] << 8 | tc[2] << 16 | tc[3] << 24, tc[4] | tc[5] << 8 | mesh << 24));
+		t->rows.push_back(make_uint2(tc[0] | tc[1] << 8 | tc[2] << 16 | tc[3] << 24,
+		                             tc[4] | tc[5] << 8 | self_show | mesh << 24));
 		t->pid_vox.push_back((dp ? kVoxDirect : 0u) | (ip ? kVoxIndirect : 0u) | cls << kVoxClassShift | flags |
 		                     pid << kVoxPidShift);
 	};
@@ -173,6 +193,13 @@ constexpr uint32_t kPlainRecs = 64;
 // a lone workgroup has the GPU to itself, so more lanes per chunk shorten it (cave chunks: 23.9 us at 512
 // threads, 27.7 us at 256, 37.3 us at 128). The persistent batch keeps its tuned shape.
 constexpr int kChunkLaunchThreads = 512;
+// A one-chunk call copies back this many quads with its result, records and vertices in one go; a chunk
+// with more takes a second round trip.
+constexpr uint32_t kPlainFastQuads = 2048;
+constexpr size_t kHaloBytes = HCM_HALO_BLOCKS * 2 + HCM_HALO_LIGHT;
+constexpr size_t kPoutRecs = 64, kPoutVerts = 64 + kPlainRecs * sizeof(hcm_mesh_rec);
+constexpr size_t kPoutIdx = kPoutVerts + kPlainFastQuads * 4 * sizeof(uint2);
+constexpr size_t kPoutBytes = kPoutIdx + kPlainFastQuads * 6 * 2;
 
 } // namespace
 
@@ -206,6 +233,11 @@ struct hcm_ctx {
 	hcm_job_result *d_pres = nullptr;
 	uint16_t *h_pblocks = nullptr; // pinned staging
 	uint8_t *h_plight = nullptr;
+	// plain path from a packed halo: the halo on the device and in pinned memory, and a pinned copy of
+	// the result, the mesh records and the first kPlainFastQuads quads
+	uint8_t *d_halo_in = nullptr;
+	uint8_t *h_halo_in = nullptr;
+	uint8_t *h_pout = nullptr;
 	// plain path light rebuild
 	int8_t *d_pheights = nullptr;
 	int8_t *h_pheights = nullptr;
@@ -307,6 +339,11 @@ static void free_ctx(hcm_ctx *c) {
 		(void)hipHostFree(c->h_pblocks);
 	if (c->h_plight)
 		(void)hipHostFree(c->h_plight);
+	dfree(c->d_halo_in);
+	if (c->h_halo_in)
+		(void)hipHostFree(c->h_halo_in);
+	if (c->h_pout)
+		(void)hipHostFree(c->h_pout);
 	dfree(c->d_pheights);
 	dfree(c->d_ppacked);
 	dfree(c->d_plevels);
@@ -380,6 +417,9 @@ static int create_impl(hcm_ctx *c, const hcm_registry *registry, uint64_t pool_q
 	HCM_HIP(dmalloc(&c->d_pres, 1));
 	HCM_HIP(hipHostMalloc(reinterpret_cast<void **>(&c->h_pblocks), 27 * HCM_CHUNK_VOXELS * 2));
 	HCM_HIP(hipHostMalloc(reinterpret_cast<void **>(&c->h_plight), 27 * HCM_CHUNK_VOXELS));
+	HCM_HIP(dmalloc(&c->d_halo_in, kHaloBytes));
+	HCM_HIP(hipHostMalloc(reinterpret_cast<void **>(&c->h_halo_in), kHaloBytes));
+	HCM_HIP(hipHostMalloc(reinterpret_cast<void **>(&c->h_pout), kPoutBytes));
 	HCM_HIP(dmalloc(&c->d_pheights, 27 * 256));
 	HCM_HIP(hipHostMalloc(reinterpret_cast<void **>(&c->h_pheights), 27 * 256));
 	HCM_HIP(dmalloc(&c->d_ppacked, 27 * kLightPlanes * 256));
@@ -504,21 +544,15 @@ static int fetch_job(hipStream_t stream, const hcm_job_result &res, const hcm_me
 	return HCM_OK;
 }
 
-extern "C" int hcm_mesh_chunk(hcm_ctx *ctx, const uint16_t *const blocks[27], const uint8_t *const light[27],
-                              hcm_host_result *out) {
-	if (!ctx || !blocks || !light || !out)
-		return HCM_ERR_INVALID;
-	for (int i = 0; i < 27; ++i)
-		if (!blocks[i] || !light[i])
-			return HCM_ERR_INVALID;
-	std::lock_guard<std::mutex> lk(ctx->mu);
-	for (int i = 0; i < 27; ++i) {
-		std::memcpy(ctx->h_pblocks + (size_t)i * HCM_CHUNK_VOXELS, blocks[i], HCM_CHUNK_VOXELS * 2);
-		std::memcpy(ctx->h_plight + (size_t)i * HCM_CHUNK_VOXELS, light[i], HCM_CHUNK_VOXELS);
-	}
+// One chunk from the packed halo in ctx->h_halo_in (context lock held): one upload, the halo scattered into
+// the plain path's 27 slots on the device, one meshing workgroup, then the result, the mesh records and
+// the first kPlainFastQuads quads copied back together; the meshes of such a chunk are assembled from that
+// copy after one synchronisation, a larger chunk's are fetched mesh by mesh.
+static int mesh_halo_locked(hcm_ctx *ctx, hcm_host_result *out) {
 	hipStream_t st = ctx->stream;
-	HCM_HIP(hipMemcpyAsync(ctx->d_pblocks, ctx->h_pblocks, 27 * HCM_CHUNK_VOXELS * 2, hipMemcpyHostToDevice, st));
-	HCM_HIP(hipMemcpyAsync(ctx->d_plight, ctx->h_plight, 27 * HCM_CHUNK_VOXELS, hipMemcpyHostToDevice, st));
+	HCM_HIP(hipMemcpyAsync(ctx->d_halo_in, ctx->h_halo_in, kHaloBytes, hipMemcpyHostToDevice, st));
+	HCM_HIP(launch_halo_unpack(reinterpret_cast<const uint16_t *>(ctx->d_halo_in), ctx->d_halo_in + HCM_HALO_BLOCKS * 2,
+	                           ctx->d_pblocks, ctx->d_plight, st));
 	HCM_HIP(launch_reset(ctx->d_pctr, st));
 	MeshParams p = base_params(ctx);
 	p.blocks = ctx->d_pblocks;
@@ -535,10 +569,88 @@ extern "C" int hcm_mesh_chunk(hcm_ctx *ctx, const uint16_t *const blocks[27], co
 	p.rec_cap = kPlainRecs;
 	p.results = ctx->d_pres;
 	HCM_HIP(launch_mesh(p, kChunkLaunchThreads, 1, st));
-	hcm_job_result res;
-	HCM_HIP(hipMemcpyAsync(&res, ctx->d_pres, sizeof(res), hipMemcpyDeviceToHost, st));
+	uint8_t *po = ctx->h_pout;
+	HCM_HIP(hipMemcpyAsync(po, ctx->d_pres, sizeof(hcm_job_result), hipMemcpyDeviceToHost, st));
+	HCM_HIP(hipMemcpyAsync(po + kPoutRecs, ctx->d_precs, kPlainRecs * sizeof(hcm_mesh_rec), hipMemcpyDeviceToHost, st));
+	HCM_HIP(hipMemcpyAsync(po + kPoutVerts, ctx->d_pverts, kPlainFastQuads * 4 * sizeof(uint2), hipMemcpyDeviceToHost, st));
+	HCM_HIP(hipMemcpyAsync(po + kPoutIdx, ctx->d_pidx, kPlainFastQuads * 6 * 2, hipMemcpyDeviceToHost, st));
 	HCM_HIP(hipStreamSynchronize(st));
-	return fetch_job(st, res, ctx->d_precs, ctx->d_pverts, ctx->d_pidx, out);
+	hcm_job_result res;
+	std::memcpy(&res, po, sizeof(res));
+	if (res.status != HCM_OK)
+		return res.status;
+	const hcm_mesh_rec *recs = reinterpret_cast<const hcm_mesh_rec *>(po + kPoutRecs);
+	bool fits = res.first_mesh + res.mesh_count <= kPlainRecs;
+	for (uint32_t m = 0; fits && m < res.mesh_count; ++m)
+		fits = (uint64_t)recs[res.first_mesh + m].first_quad + recs[res.first_mesh + m].quad_count <= kPlainFastQuads;
+	if (!fits)
+		return fetch_job(st, res, ctx->d_precs, ctx->d_pverts, ctx->d_pidx, out);
+	uint64_t nv = 0, ni = 0;
+	for (uint32_t m = 0; m < res.mesh_count; ++m)
+		nv += 4ull * recs[res.first_mesh + m].quad_count, ni += 6ull * recs[res.first_mesh + m].quad_count;
+	out->vertex_count = (uint32_t)nv;
+	out->index_count = (uint32_t)ni;
+	out->mesh_count = res.mesh_count;
+	if (nv > out->vertex_capacity || ni > out->index_capacity || res.mesh_count > out->mesh_capacity ||
+	    (nv && (!out->vertices || !out->indices)) || (res.mesh_count && !out->meshes))
+		return HCM_ERR_CAPACITY;
+	const uint2 *verts = reinterpret_cast<const uint2 *>(po + kPoutVerts);
+	const uint16_t *idx = reinterpret_cast<const uint16_t *>(po + kPoutIdx);
+	uint32_t vb = 0, ib = 0;
+	for (uint32_t m = 0; m < res.mesh_count; ++m) {
+		const hcm_mesh_rec &r = recs[res.first_mesh + m];
+		hcm_mesh_desc &d = out->meshes[m];
+		d.vert_begin = vb, d.vert_count = 4 * r.quad_count;
+		d.idx_begin = ib, d.idx_count = 6 * r.quad_count;
+		std::memcpy(d.aabb, r.aabb, sizeof(d.aabb));
+		d.transparent = r.transparent;
+		std::memcpy(out->vertices + vb, verts + 4ull * r.first_quad, 4ull * r.quad_count * sizeof(uint2));
+		std::memcpy(out->indices + ib, idx + 6ull * r.first_quad, 6ull * r.quad_count * 2);
+		vb += d.vert_count, ib += d.idx_count;
+	}
+	return HCM_OK;
+}
+
+extern "C" int hcm_mesh_halo(hcm_ctx *ctx, const uint16_t *blocks, const uint8_t *light, hcm_host_result *out) {
+	if (!ctx || !blocks || !light || !out)
+		return HCM_ERR_INVALID;
+	std::lock_guard<std::mutex> lk(ctx->mu);
+	std::memcpy(ctx->h_halo_in, blocks, HCM_HALO_BLOCKS * 2);
+	std::memcpy(ctx->h_halo_in + HCM_HALO_BLOCKS * 2, light, HCM_HALO_LIGHT);
+	return mesh_halo_locked(ctx, out);
+}
+
+extern "C" int hcm_mesh_chunk(hcm_ctx *ctx, const uint16_t *const blocks[27], const uint8_t *const light[27],
+                              hcm_host_result *out) {
+	if (!ctx || !blocks || !light || !out)
+		return HCM_ERR_INVALID;
+	for (int i = 0; i < 27; ++i)
+		if (!blocks[i] || !light[i])
+			return HCM_ERR_INVALID;
+	std::lock_guard<std::mutex> lk(ctx->mu);
+	// the packed halo, row by row: an x-row spans the chunks at cx = -1, 0, 1 (neighbour-slot order of
+	// Position.hpp:138-145: L(-1) = 1, L(0) = 2, L(1) = 0)
+	uint16_t *hb = reinterpret_cast<uint16_t *>(ctx->h_halo_in);
+	uint8_t *hl = ctx->h_halo_in + HCM_HALO_BLOCKS * 2;
+	auto lut = [](int v) { return v < 0 ? 1 : (v >= 16 ? 0 : 2); };
+	for (int y = -1; y < 17; ++y)
+		for (int z = -2; z < 18; ++z) {
+			const int yz = 3 * lut(y) + lut(z);
+			const size_t row = 16u * (size_t)((z & 15) + 16 * (y & 15));
+			uint16_t *dst = hb + 20 * ((z + 2) + 20 * (y + 1));
+			const uint16_t *lo = blocks[9 * 1 + yz] + row, *mid = blocks[9 * 2 + yz] + row, *hi = blocks[yz] + row;
+			dst[0] = lo[14], dst[1] = lo[15];
+			std::memcpy(dst + 2, mid, 32);
+			dst[18] = hi[0], dst[19] = hi[1];
+			if (z >= -1 && z < 17) {
+				uint8_t *ld = hl + 18 * ((z + 1) + 18 * (y + 1));
+				const uint8_t *llo = light[9 * 1 + yz] + row, *lmid = light[9 * 2 + yz] + row, *lhi = light[yz] + row;
+				ld[0] = llo[15];
+				std::memcpy(ld + 1, lmid, 16);
+				ld[17] = lhi[0];
+			}
+		}
+	return mesh_halo_locked(ctx, out);
 }
 
 extern "C" int hcm_mesh_chunk_relight(hcm_ctx *ctx, const uint16_t *const blocks[27], const int8_t *const heights[27],

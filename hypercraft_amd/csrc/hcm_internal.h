@@ -12,13 +12,21 @@ constexpr int kMaxPid = 128;   // registry rows + dynamic liquid-texture rows ke
 constexpr int kMaxCodes = 128; // distinct face textures kept in LDS
 constexpr int kFlowPatterns = 17;
 
-// Voxel word staged in LDS per halo voxel: the light byte, then the per-row template below.
-constexpr uint32_t kVoxDirect = 1u << 8;   // VerticalLightPass || collision != solid (BlockMeshAlgo.hpp:119)
-constexpr uint32_t kVoxIndirect = 1u << 9; // transparency != opaque (block/include/block/Block.hpp:70)
-constexpr int kVoxClassShift = 10;         // 2-bit face class per BlockFace f at bit 10 + 2 f
-constexpr uint32_t kVoxMesh = 1u << 22;    // has a custom mesh
-constexpr uint32_t kVoxFlow = 1u << 23;    // liquid flow (dynamic mesh and face textures)
-constexpr int kVoxPidShift = 24;           // texture row
+// Voxel word staged in LDS per halo voxel: sunlight in bits 0-3 and torchlight in bits 16-19 (the light
+// byte's low and high nibbles), and the per-row template below. With the indirect-pass bit at 12, a cell's
+// contribution to a light4_init corner sum is the word masked to kVoxLight when it passes light: adding
+// such words sums sunlight in bits 0-5, the count of cells in bits 12-13 and torchlight in bits 16-21.
+constexpr int kVoxPidShift = 4;            // texture row, 7 bits
+constexpr uint32_t kVoxPidMask = 0x7fu;
+constexpr uint32_t kVoxDirect = 1u << 11;   // VerticalLightPass || collision != solid (BlockMeshAlgo.hpp:119)
+constexpr uint32_t kVoxIndirect = 1u << 12; // transparency != opaque (block/include/block/Block.hpp:70)
+constexpr uint32_t kVoxMesh = 1u << 13;     // has a custom mesh
+constexpr uint32_t kVoxFlow = 1u << 14;     // liquid flow (dynamic mesh and face textures)
+constexpr int kVoxClassShift = 20;          // 2-bit face class per BlockFace f at bit 20 + 2 f
+constexpr uint32_t kVoxSunTorch = 0x000f000fu;
+constexpr uint32_t kVoxLight = kVoxSunTorch | kVoxIndirect;
+// The light byte (sun | torch << 4) moved to the word's light fields.
+__host__ __device__ constexpr uint32_t vox_light(uint32_t byte) { return (byte * 0x1001u) & kVoxSunTorch; }
 // Face classes for BlockTexture::Show (texture/include/texture/BlockTexture.hpp:68-76)
 constexpr uint32_t kClassEmpty = 0, kClassOpaque = 1, kClassTransparent = 2, kClassLiquid = 3;
 
@@ -28,11 +36,15 @@ constexpr uint32_t kCodeTransPass = 2u;
 constexpr uint32_t kCodeLiquid = 4u;
 constexpr uint32_t kCodeEmpty = 8u;
 
+// Texture-row flag (in rows[pid].y): a halo made of this row's block alone has visible faces.
+constexpr uint32_t kRowSelfShow = 1u << 16;
+
 struct DevReg {
 	const uint8_t *pid_of;         // [65536] Block::m_data -> texture row (pid); then [256] by block id:
 	                               // the row when the meta byte does not change it, else 0xff
-	const uint2 *rows;             // [n_pid]: x = tc0..tc3 (8 bits each), y = tc4 | tc5 << 8 | mesh << 24
-	const uint32_t *pid_vox;       // [n_pid]: voxel-word template (bits 8..31 above)
+	const uint2 *rows;             // [n_pid]: x = tc0..tc3 (8 bits each),
+	                               // y = tc4 | tc5 << 8 | kRowSelfShow | mesh << 24
+	const uint32_t *pid_vox;       // [n_pid]: voxel-word template (everything but the light fields)
 	const uint32_t *codes;         // [n_codes]
 	const hcm_custom_mesh *meshes; // [n_meshes]
 	const uint8_t *tex_flags;      // [n_tex] by texture id
@@ -114,6 +126,8 @@ struct VoxelEdit {
 // Launchers (hcm_kernels.hip)
 hipError_t launch_reset(Counters *ctr, hipStream_t stream);
 hipError_t launch_mesh(const MeshParams &p, int threads, int grid, hipStream_t stream);
+hipError_t launch_halo_unpack(const uint16_t *hblocks, const uint8_t *hlight, uint16_t *blocks, uint8_t *light,
+                              hipStream_t stream);
 hipError_t launch_edits(uint16_t *blocks, uint8_t *light, int8_t *heights, const VoxelEdit *edits, uint32_t n,
                         hipStream_t stream);
 int mesh_lds_bytes(int threads);

@@ -5,8 +5,9 @@
 //   1. staging: one thread per 18-voxel x-row of the 18^3 halo loads the row from its chunk slots with
 //      vector loads and writes one LDS word per voxel: light byte, direct/indirect light-pass bits,
 //      a 2-bit face class per face (empty / opaque / transparent / liquid), custom-mesh and flow bits
-//      and the texture row. A chunk without a custom mesh whose halo holds a single block value or only
-//      opaque cubes, or none of whose own voxels has a face, has no quads and stops here; so does one
+//      and the texture row. A chunk without a custom mesh whose halo holds a single block value (whose
+//      opposite faces hide each other) or only opaque cubes, or none of whose own voxels has a face, has
+//      no quads and stops here; so does one
 //      with no visible face after step 3's visibility masks.
 //   2. liquid-flow cells take their per-cell face textures (the reference's m_dynamic_block_textures).
 //   3. faces, per (axis, slice, direction) task, each a 16x16 mask:
@@ -111,14 +112,11 @@ template <int NT> struct Smem {
 	alignas(16) uint16_t eh[kTasks][16];
 	alignas(16) uint16_t ev[kTasks][16];
 	alignas(16) uint16_t tp[kTasks][16];
-	union {
-		uint2 key[kListCap];  // face words of the current group of tasks, in dense order
-		uint2 list[kListCap]; // quad descriptors of the current emission window
-	};
-	uint32_t tpre[kTasks + 2];  // prefix of visible faces per task; later of quads in emission order
-	uint32_t qpre_o[kTasks];     // per task: index of its first opaque quad in the opaque pass
-	uint32_t rpre[kTasks + 1];   // prefix over tasks of their mask rows holding quad starts
-	uint16_t rmask[kTasks];      // per task: the mask rows holding quad starts
+	uint2 key[kListCap];       // face words of the current group of tasks, in dense order
+	uint32_t fpre[kTasks + 2]; // prefix of visible faces per task (the dense face order)
+	uint32_t qpre_o[kTasks];   // per task: index of its first opaque quad in the opaque pass
+	uint32_t qpre_t[kTasks];   // per task: index of its first transparent quad in the transparent pass
+	uint32_t epre[kTasks];     // per task: position of its first quad in the emission order (mesh splits)
 	uint2 rows[kMaxPid];
 	uint32_t pid_vox[kMaxPid];
 	uint32_t codes[kMaxCodes];
@@ -139,12 +137,14 @@ constexpr uint32_t kFlagNonUniform = 1u, kFlagCustom = 2u, kFlagFlow = 4u;
 // without either has no visible face: BlockTexture::Show never shows an empty face, nor an opaque face
 // against an opaque one (texture/include/texture/BlockTexture.hpp:68-76).
 constexpr uint32_t kFlagMixed = 8u, kFlagInnerFaces = 16u;
-constexpr uint32_t kVoxShape = 0xfffc00u;                      // the face classes and the mesh and flow bits
+constexpr uint32_t kVoxShape = 0xfff00000u | kVoxMesh | kVoxFlow; // the face classes and the mesh and flow bits
 constexpr uint32_t kVoxOpaqueCube = 0x555u << kVoxClassShift; // opaque on all six faces
 
 // ------------------------------------------------------------------------------------------------
 // small helpers
 // ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t vox_pid(uint32_t vox) { return (vox >> kVoxPidShift) & kVoxPidMask; }
+
 template <int NT> __device__ __forceinline__ uint32_t row_code(const Smem<NT> &s, uint32_t pid, int face) {
 	const uint2 r = s.rows[pid];
 	const uint32_t w = face < 4 ? r.x : r.y;
@@ -159,12 +159,6 @@ template <int NT> __device__ __forceinline__ uint32_t block_pid(const MeshParams
 
 __device__ __forceinline__ uint32_t face_class(uint32_t vox, int face) {
 	return (vox >> (kVoxClassShift + 2 * face)) & 3u;
-}
-
-// (s << 2) / cnt for cnt in 1..4 and s <= 60, exactly: multiply by ceil(65536 / cnt) and shift.
-__device__ __forceinline__ uint32_t div_cnt(uint32_t x, uint32_t cnt) {
-	const uint32_t r = cnt == 1 ? 65536u : (cnt == 2 ? 32768u : (cnt == 3 ? 21846u : 16384u));
-	return (x * r) >> 16;
 }
 
 // AO of a light4_init vertex (:126-130) by side, corner and side cell direct-pass bits and the front cell's:
@@ -182,26 +176,40 @@ constexpr uint32_t ao_table() {
 constexpr uint32_t kAoTable = ao_table();
 static_assert(kAoTable == 0xe9849040u, "AO table");
 
-// One vertex of light4_init (:114-162) from its side, corner and side cells and the front cell, without
-// branches: the sunlight and torchlight nibbles are summed as one packed word (sunlight in bits 0-7,
-// torchlight in bits 16-23), a side or corner cell counting only if it passes indirect light and one of
-// the side cells does (:132-161).
-__device__ __forceinline__ void light_vertex(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t pass3, uint32_t fs,
-                                             uint32_t ft, uint32_t &ao, uint32_t &sun, uint32_t &torch) {
-	const uint32_t idx = ((c0 >> 8) & 1u) | ((c1 >> 7) & 2u) | ((c2 >> 6) & 4u) | (pass3 << 3);
-	ao = (kAoTable >> (2 * idx)) & 3u;
-	const uint32_t g = ((c0 | c2) >> 9) & 1u;
-	const uint32_t m0 = 0u - (((c0 >> 9) & 1u) & g), m1 = 0u - (((c1 >> 9) & 1u) & g),
-	               m2 = 0u - (((c2 >> 9) & 1u) & g);
-	const uint32_t sum = (fs | (ft << 16)) + (((c0 & 15u) | ((c0 & 0xf0u) << 12)) & m0) +
-	                     (((c1 & 15u) | ((c1 & 0xf0u) << 12)) & m1) + (((c2 & 15u) | ((c2 & 0xf0u) << 12)) & m2);
-	const uint32_t cnt = 1u + (m0 & 1u) + (m1 & 1u) + (m2 & 1u);
-	sun = div_cnt((sum & 0xffu) << 2, cnt);
-	torch = div_cnt((sum >> 16) << 2, cnt);
+// (sum << 2) / cnt for cnt in 1..4 and sum <= 15 cnt, exactly: (sum * R) >> 7 with R = 512, 256, 171, 128,
+// packed 16 bits apart by cnt - 1. Checked below for every possible sum.
+constexpr uint64_t kRecip = 0x008000ab01000200ull;
+constexpr bool recip_exact() {
+	for (uint32_t cnt = 1; cnt <= 4; ++cnt)
+		for (uint32_t sum = 0; sum <= 15 * cnt; ++sum)
+			if (((sum * uint32_t((kRecip >> (16 * (cnt - 1))) & 0x3ffu)) >> 7) != (sum << 2) / cnt)
+				return false;
+	return true;
+}
+static_assert(recip_exact(), "reciprocal table");
+
+// A cell's light in a light4_init sum: its word masked to the light fields and the count bit when it
+// passes indirect light (:132-161), else 0.
+__device__ __forceinline__ uint32_t lpass(uint32_t c) { return c & kVoxLight & (0u - ((c >> 12) & 1u)); }
+
+// One vertex of light4_init (:114-162), without branches, from its side cells a and b, corner cell c and
+// the front cell: the side cells count when they pass indirect light, the corner when it does and a side
+// cell does; sunlight and torchlight are averaged as (sum << 2) / count, the front cell always counted.
+// Returns sun | torch << 8 | ao << 16.
+__device__ __forceinline__ uint32_t light_vertex(uint32_t a, uint32_t c, uint32_t b, uint32_t pf, uint32_t d3) {
+	const uint32_t g = 0u - (((a | b) >> 12) & 1u);
+	const uint32_t sum = pf + lpass(a) + lpass(b) + (lpass(c) & g); // count - 1 in bits 12-13
+	const uint32_t r = uint32_t(kRecip >> ((sum >> 8) & 0x30u)) & 0x3ffu;
+	const uint32_t prod = (sum & 0x003f003fu) * r;
+	const uint32_t idx = ((a >> 11) & 1u) | ((c >> 10) & 2u) | ((b >> 9) & 4u) | d3;
+	return ((prod >> 7) & 63u) | (((prod >> 23) & 63u) << 8) | (((kAoTable >> (2 * idx)) & 3u) << 16);
 }
 
 // light4_init (:48-163) for face `face` of the block at halo index self. Returns the packed Light4:
 //   x = sun0 | sun1 << 6 | sun2 << 12 | sun3 << 18 | ao4 << 24,   y = torch0 | ... | torch3 << 18.
+// The front cell F and its eight neighbours in the face's plane; vertex k's side cells along the plane
+// axes Q and P and the corner between them have (P, Q) signs (-,-), then (+,-) or (-,+) by the winding,
+// then (+,+), then the other of (+,-) and (-,+).
 __device__ __forceinline__ uint2 light4(const uint32_t *vox, int face, int self) {
 	const int a = face >> 1;
 	const int sn = a == 0 ? kSX : (a == 1 ? kSY : kSZ);
@@ -212,22 +220,15 @@ __device__ __forceinline__ uint2 light4(const uint32_t *vox, int face, int self)
 	const uint32_t qm = vox[f - sq], qp = vox[f + sq], pm = vox[f - sp], pp = vox[f + sp];
 	const uint32_t mm = vox[f - sp - sq], mp = vox[f - sp + sq], pmq = vox[f + sp - sq], ppq = vox[f + sp + sq];
 	const bool wa = face == 0 || face == 3 || face == 4;
-	const uint32_t pass3 = (F >> 8) & 1u, fs = F & 15u, ft = (F >> 4) & 15u;
-	uint32_t lo = 0, hi = 0, ao4 = 0;
-	// (P, Q) signs (-,-), then (+,-) or (-,+) by the winding, then (+,+), then the other of (+,-) and (-,+)
-#pragma unroll
-	for (int k = 0; k < 4; ++k) {
-		const bool pplus = k == 2 || (k == 1 && wa) || (k == 3 && !wa);
-		const bool qplus = k == 2 || (k == 1 && !wa) || (k == 3 && wa);
-		const uint32_t c0 = qplus ? qp : qm;                                  // side cell along Q
-		const uint32_t c1 = pplus ? (qplus ? ppq : pmq) : (qplus ? mp : mm); // corner cell
-		const uint32_t c2 = pplus ? pp : pm;                                  // side cell along P
-		uint32_t ao, sun, torch;
-		light_vertex(c0, c1, c2, pass3, fs, ft, ao, sun, torch);
-		lo |= sun << (6 * k);
-		hi |= torch << (6 * k);
-		ao4 |= ao << (2 * k);
-	}
+	const uint32_t pf = F & kVoxSunTorch, d3 = ((F >> 11) & 1u) << 3;
+	// vertex 1 is corner (+,-) with winding a, else (-,+); vertex 3 the other
+	const uint32_t a1 = wa ? qm : qp, c1 = wa ? pmq : mp, b1 = wa ? pp : pm;
+	const uint32_t a3 = wa ? qp : qm, c3 = wa ? mp : pmq, b3 = wa ? pm : pp;
+	const uint32_t v0 = light_vertex(qm, mm, pm, pf, d3), v1 = light_vertex(a1, c1, b1, pf, d3);
+	const uint32_t v2 = light_vertex(qp, ppq, pp, pf, d3), v3 = light_vertex(a3, c3, b3, pf, d3);
+	const uint32_t lo = (v0 & 63u) | (v1 & 63u) << 6 | (v2 & 63u) << 12 | (v3 & 63u) << 18;
+	const uint32_t hi = ((v0 >> 8) & 63u) | ((v1 >> 8) & 63u) << 6 | ((v2 >> 8) & 63u) << 12 | ((v3 >> 8) & 63u) << 18;
+	const uint32_t ao4 = (v0 >> 16) | (v1 >> 16) << 2 | (v2 >> 16) << 4 | (v3 >> 16) << 6;
 	return make_uint2(lo | (ao4 << 24), hi);
 }
 
@@ -513,19 +514,14 @@ __device__ __forceinline__ void aabb_merge(Smem<NT> &s, int pass, int k, const u
 }
 
 // Regular quad: task t = 34 * axis + 2 * slice + direction, origin (u, v0) on the mask, width w along V,
-// height h along U (:357-460).
+// height h along U (:357-460), with the face word of its origin face (Light4 and texture code).
 template <int NT>
-__device__ __forceinline__ void emit_regular(const MeshParams &p, Smem<NT> &s, uint32_t desc, uint32_t slot,
-                                          uint32_t vbase, uint32_t lo[3], uint32_t hi[3]) {
-	const int t = desc & 127, u = (desc >> 7) & 15, v0 = (desc >> 11) & 15;
-	const int w = int((desc >> 15) & 15) + 1, h = int((desc >> 19) & 15) + 1;
+__device__ __forceinline__ void emit_quad(const MeshParams &p, const Smem<NT> &s, int t, int u, int v0, int w, int h,
+                                          uint2 word, uint32_t slot, uint32_t vbase, uint32_t lo[3], uint32_t hi[3]) {
 	const int A = t / 34, sl = (t % 34) >> 1, inv = t & 1, face = 2 * A + inv;
 	const int ua = u_axis(A), va = v_axis(A);
-	const int layer = inv ? sl : sl - 1;
-	const int self = (layer + 1) * axis_stride(A) + (u + 1) * axis_stride(ua) + (v0 + 1) * axis_stride(va);
-	const uint2 l = light4(s.vox, face, self);
-	const uint32_t tc = row_code(s, s.vox[self] >> kVoxPidShift, face);
-	const uint16_t tex = uint16_t(s.codes[tc] & 0xffffu);
+	const uint2 l = make_uint2(word.x, word.y & 0xffffffu);
+	const uint16_t tex = uint16_t(s.codes[(word.y >> 24) & 0x7fu] & 0xffffu);
 	// quad vectors (:383-390): (U+1)%3 == V holds for the x and y axes
 	const bool height_on_u = (inv != 0) != (A != 2);
 	int du[3], dv[3], o[3];
@@ -672,7 +668,7 @@ template <int A, int NT> __device__ __forceinline__ void visibility_column(Smem<
 		for (int a = 0; a <= 16; ++a) {
 			const uint32_t wa = cw[a * sa], wb = cw[(a + 1) * sa];
 			const bool d = (((wa >> shp) & 3u) | ((wb >> shp) & 12u)) == kShowTT &&
-			               row_code(s, wa >> kVoxPidShift, 2 * A) != row_code(s, wb >> kVoxPidShift, 2 * A + 1);
+			               row_code(s, vox_pid(wa), 2 * A) != row_code(s, vox_pid(wb), 2 * A + 1);
 			const uint64_t bd = ballot64(d);
 			if (bd && lane == 0) {
 				if (a >= 1)
@@ -692,24 +688,21 @@ __device__ __forceinline__ int face_owner(int t, int u, int v, int &face) {
 	return (layer + 1) * axis_stride(A) + (u + 1) * axis_stride(u_axis(A)) + (v + 1) * axis_stride(v_axis(A));
 }
 
-// Word of the visible face with dense index gi (task-major, row-major, as the reference scans them):
-// Light4 (light4_init, :48-163) in x and the low 24 bits of y, the texture code in y bits 24-30 and the
-// transparent-pass flag in bit 31. Faces merge only if their words are equal (Light4::operator==, :44,
-// and the texture compare, :367-368).
-template <int NT>
-__device__ __forceinline__ uint2 face_word(const Smem<NT> &s, int t0, int t1, uint32_t gi, uint32_t &desc) {
-	int lo = t0, hi = t1 - 1; // the task holding gi: tpre[t] <= gi < tpre[t + 1]
+// Task, mask row and column of the visible face with dense index gi (task-major, row-major, as the
+// reference scans them), searched among the tasks [t0, t1).
+template <int NT> __device__ __forceinline__ void face_pos(const Smem<NT> &s, int t0, int t1, uint32_t gi, int &t, int &u, int &v) {
+	int lo = t0, hi = t1 - 1; // the task holding gi: fpre[t] <= gi < fpre[t + 1]
 	while (lo < hi) {
 		const int mid = (lo + hi + 1) >> 1;
-		if (s.tpre[mid] <= gi)
+		if (s.fpre[mid] <= gi)
 			lo = mid;
 		else
 			hi = mid - 1;
 	}
-	const int t = lo;
-	uint32_t rem = gi - s.tpre[t];
+	t = lo;
+	uint32_t rem = gi - s.fpre[t];
 	const uint64_t *rows = reinterpret_cast<const uint64_t *>(s.ne[t]);
-	int u = 0;
+	u = 0;
 	uint32_t bits = 0;
 #pragma unroll 1
 	for (int q = 0; q < 4; ++q) { // the row holding it
@@ -730,7 +723,7 @@ __device__ __forceinline__ uint2 face_word(const Smem<NT> &s, int t0, int t1, ui
 		break;
 	}
 	// the column: the rem-th set bit of the row
-	int v = 0;
+	v = 0;
 	const uint32_t c8 = __popc(bits & 0xffu);
 	if (rem >= c8)
 		rem -= c8, bits >>= 8, v += 8;
@@ -742,11 +735,21 @@ __device__ __forceinline__ uint2 face_word(const Smem<NT> &s, int t0, int t1, ui
 		rem -= c2, bits >>= 2, v += 2;
 	if (rem >= (bits & 1u))
 		v += 1;
+}
+
+// Word of the visible face with dense index gi (task-major, row-major, as the reference scans them):
+// Light4 (light4_init, :48-163) in x and the low 24 bits of y, the texture code in y bits 24-30 and the
+// transparent-pass flag in bit 31. Faces merge only if their words are equal (Light4::operator==, :44,
+// and the texture compare, :367-368).
+template <int NT>
+__device__ __forceinline__ uint2 face_word(const Smem<NT> &s, int t0, int t1, uint32_t gi, uint32_t &desc) {
+	int t, u, v;
+	face_pos(s, t0, t1, gi, t, u, v);
 	desc = uint32_t(t) | uint32_t(u) << 7 | uint32_t(v) << 11;
 	int face;
 	const int self = face_owner(t, u, v, face);
 	const uint2 l = light4(s.vox, face, self);
-	const uint32_t tc = row_code(s, s.vox[self] >> kVoxPidShift, face);
+	const uint32_t tc = row_code(s, vox_pid(s.vox[self]), face);
 	const uint32_t tpass = (s.codes[tc] >> 16) & kCodeTransPass ? 1u : 0u;
 	return make_uint2(l.x, l.y | (tc << 24) | (tpass << 31));
 }
@@ -783,11 +786,6 @@ template <int NT> __device__ __forceinline__ void face_links(Smem<NT> &s, uint32
 // in row u are therefore exactly the maximal runs of uncovered visible cells split wherever the face word
 // changes, and each run's height depends only on the rows below. Here a row's runs are found with bit
 // operations and all of them are extended downward together.
-struct Quad {
-	uint32_t u, v0, w, h, trans;
-	uint32_t ord, ord_pass; // index within the task, and within the task's quads of the same pass
-};
-
 // Moves marked bits from v to v - 1 (spread_down) or v - 1 to v (spread_up) while v is in `link`, i.e.
 // while v continues the run of v - 1: it reaches a run's start from any of its cells, and vice versa.
 __device__ __forceinline__ uint32_t spread_down(uint32_t d, uint32_t link) {
@@ -825,16 +823,13 @@ __device__ __forceinline__ void greedy_count(Smem<NT> &s, int t, bool active, ui
 		ev[k] = active ? evs[k] : 0u;
 	}
 	n_o = 0, n_t = 0;
-	uint32_t rows = 0; // rows holding quad starts
 #pragma unroll 1
 	for (int u = 0; u < 16; ++u) {
 		const uint32_t ne = active ? s.ne[t][u] : 0u, eh = active ? s.eh[t][u] : 0u, tp = active ? s.tp[t][u] : 0u;
 		const uint32_t S = ne & ~(cov[0] & 0xffffu); // uncovered visible cells of row u
 		const uint32_t link = S & ((S & eh) << 1);   // v continues the run of v - 1
 		const uint32_t starts = S & ~link;
-		rows |= (starts ? 1u : 0u) << u;
 		if (active) { // row u's results replace its (already read) inputs; ev's rows are in registers
-			s.ne[t][u] = uint16_t(ne & ~S);     // cells covered by quads of earlier rows
 			s.ev[t][u] = uint16_t(S);
 			s.eh[t][u] = uint16_t(starts & ~tp); // opaque quads start here
 			s.tp[t][u] = uint16_t(starts & tp);  // transparent quads start here
@@ -873,68 +868,111 @@ __device__ __forceinline__ void greedy_count(Smem<NT> &s, int t, bool active, ui
 			ev[k] = (ev[k] >> 16) | (nv << 16);
 		}
 	}
-	if (active) {
-		s.rmask[t] = uint16_t(rows);
-		// the covered cells, transposed to columns (bit r of column v = row r), for the quad heights:
-		// block swaps of 8, 4, 2 and 1 rows and columns
-		uint32_t a[16];
-#pragma unroll
-		for (int r = 0; r < 16; ++r)
-			a[r] = s.ne[t][r];
-#pragma unroll
-		for (int j = 8, m = 0x00ff; j != 0; j >>= 1, m ^= m << j) {
-#pragma unroll
-			for (int k = 0; k < 16; ++k) {
-				if (k & j)
-					continue;
-				const uint32_t x = ((a[k] >> j) ^ a[k + j]) & uint32_t(m);
-				a[k] ^= x << j;
-				a[k + j] ^= x;
-			}
+}
+
+// ------------------------------------------------------------------------------------------------
+// emission of one quad
+// ------------------------------------------------------------------------------------------------
+// The end of the longest run of tasks from t0 whose visible faces fit the face-word stage (one task, at
+// most 256 faces, always fits). Thread 0 searches; a barrier publishes the result.
+template <int NT> __device__ __forceinline__ int group_end(Smem<NT> &s, int t0) {
+	if (threadIdx.x == 0) {
+		int lo = t0 + 1, hi = kTasks; // largest t1 with fpre[t1] - fpre[t0] <= kListCap
+		while (lo < hi) {
+			const int mid = (lo + hi + 1) >> 1;
+			if (s.fpre[mid] - s.fpre[t0] <= (uint32_t)kListCap)
+				lo = mid;
+			else
+				hi = mid - 1;
 		}
+		s.group_end = lo;
+	}
+	__syncthreads();
+	return s.group_end;
+}
+
+// Where quad i of a pass goes: its pool slot, the base vertex of its mesh, the mesh of the pass (mk) and
+// its index in that mesh (mj). The reference reads the 16-bit vertex count before splitting, so the first
+// quad of a continuation mesh keeps base 65,532 (:273-279, :397-403).
+__device__ __forceinline__ void quad_slot(uint32_t pass, uint32_t i, uint32_t n0, uint32_t pool_base, uint32_t &slot,
+                                          uint32_t &vbase, uint32_t &mk, uint32_t &mj) {
+	slot = pool_base + (pass ? n0 : 0u) + i;
+	mk = i / kSplitQuads;
+	mj = i - mk * kSplitQuads;
+	vbase = (mj == 0 && mk > 0) ? 65532u : 4u * mj;
+}
+
+// A written quad's box into its mesh's AABB (per lane for the first mesh of a pass, LDS atomics for the
+// others) and, when it starts a continuation mesh, a split event at its emission position epos.
+template <int NT>
+__device__ __forceinline__ void quad_done(Smem<NT> &s, uint32_t pass, uint32_t mk, uint32_t mj, uint32_t epos,
+                                          const uint32_t lo[3], const uint32_t hi[3], uint32_t box[2][6]) {
+	if (mk == 0) { // static register indices only
 #pragma unroll
-		for (int v = 0; v < 16; ++v)
-			s.ne[t][v] = uint16_t(a[v]);
+		for (int a = 0; a < 3; ++a) {
+			box[0][a] = pass ? box[0][a] : min(box[0][a], lo[a]);
+			box[0][3 + a] = pass ? box[0][3 + a] : max(box[0][3 + a], hi[a]);
+			box[1][a] = pass ? min(box[1][a], lo[a]) : box[1][a];
+			box[1][3 + a] = pass ? max(box[1][3 + a], hi[a]) : box[1][3 + a];
+		}
+	} else
+		aabb_merge(s, pass, mk, lo, hi);
+	if (mj == 0 && mk > 0) { // a split happened right before this quad (:397-403)
+		const uint32_t ev = atomicAdd(&s.n_events, 1u);
+		if (ev < 16) {
+			s.events[ev][0] = epos;
+			s.events[ev][1] = (pass << 16) | (mk - 1u);
+		}
 	}
 }
 
-// The quads of mask row (t, u), from what the counting pass left: C (cells covered by quads of earlier
-// rows) in ne, transposed to columns, the uncovered set S in ev, and the opaque and transparent quad
-// starts in eh and tp. A run
-// of S continues through every cell that is not a start. A quad from (u, v0) covers rows u .. u + h - 1
-// of its columns, so below it column v0 is in C exactly for rows u + 1 .. u + h - 1: the cell under its
-// last row is either not covered or starts a quad. Quads are numbered within the task in the
-// reference's order (rows, then columns) and within the task's quads of the same pass.
-template <int NT, typename Fn> __device__ __forceinline__ void describe_row(const Smem<NT> &s, int t, int u, Fn &&emit) {
-	const uint32_t so = s.eh[t][u], st_t = s.tp[t][u], starts = so | st_t;
-	if (!starts)
+// The quad that starts at the visible face with dense index gi and face word `word`, if one does. Per
+// task, ne still holds the visible cells and the counting pass left the opaque quad starts in eh, the
+// transparent ones in tp and the uncovered visible cells S in ev; the covered cells are C = ne & ~S. A
+// quad from (u, v0) spans S's run from v0 to the next start along V, and covers rows u .. u + h - 1 of its
+// columns, so below it column v0 is in C exactly for rows u + 1 .. u + h - 1: the cell under its last row
+// is either not covered or starts a quad. Its index in its pass follows the pass's starts in the task's
+// earlier rows and columns (the reference's order: rows, then columns).
+template <int NT>
+__device__ __forceinline__ void emit_face(const MeshParams &p, Smem<NT> &s, int t0, int t1, uint32_t gi, uint2 word,
+                                          uint32_t n0, uint32_t pool_base, uint32_t box[2][6]) {
+	int t, u, v;
+	face_pos(s, t0, t1, gi, t, u, v);
+	const uint32_t so = s.eh[t][u], st = s.tp[t][u], starts = so | st;
+	if (!((starts >> v) & 1u))
 		return;
-	const uint64_t *o64 = reinterpret_cast<const uint64_t *>(s.eh[t]);
-	const uint64_t *t64 = reinterpret_cast<const uint64_t *>(s.tp[t]);
-	uint32_t n_o = 0, n_t = 0; // quads of each pass in rows 0 .. u - 1
+	const uint32_t tr = (st >> v) & 1u, below = (1u << v) - 1u;
+	const uint32_t link = uint32_t(s.ev[t][u]) & ~starts;
+	const int w = __ffs(~(link >> (v + 1)));
+	int h = 1;
+#pragma unroll 1
+	for (int r = u + 1; r < 16 && ((uint32_t(s.ne[t][r] & ~s.ev[t][r]) >> v) & 1u); ++r)
+		++h;
+	const uint64_t *m64 = reinterpret_cast<const uint64_t *>(tr ? s.tp[t] : s.eh[t]);
+	uint32_t ord = __popc((tr ? st : so) & below);
 #pragma unroll
-	for (int q = 0; q < 4; ++q) {
+	for (int q = 0; q < 4; ++q) { // the pass's starts in rows 0 .. u - 1
 		const int nb = 16 * (u - 4 * q);
 		const uint64_t m = nb >= 64 ? ~0ull : (nb > 0 ? (1ull << nb) - 1ull : 0ull);
-		n_o += __popcll(o64[q] & m);
-		n_t += __popcll(t64[q] & m);
+		if (nb > 0)
+			ord += __popcll(m64[q] & m);
 	}
-	const uint32_t link = uint32_t(s.ev[t][u]) & ~starts;
-	uint32_t rest = starts;
-	while (rest) {
-		const uint32_t v0 = __ffs(rest) - 1;
-		rest &= rest - 1;
-		const uint32_t below = (1u << v0) - 1u;
-		const uint32_t tr = (st_t >> v0) & 1u;
-		// rows u + 1 .. covered in column v0 (the column's bits past row 15 are zero)
-		const uint32_t h = 1u + __builtin_ctz(~(uint32_t(s.ne[t][v0]) >> (u + 1)));
-		Quad q;
-		q.u = u, q.v0 = v0, q.trans = tr, q.h = h;
-		q.w = __ffs(~(link >> (v0 + 1)));
-		q.ord = n_o + n_t + __popc(starts & below);
-		q.ord_pass = tr ? n_t + __popc(st_t & below) : n_o + __popc(so & below);
-		emit(q);
+	const uint32_t i = (tr ? s.qpre_t[t] : s.qpre_o[t]) + ord;
+	uint32_t slot, vbase, mk, mj, lo[3], hi[3];
+	quad_slot(tr, i, n0, pool_base, slot, vbase, mk, mj);
+	emit_quad(p, s, t, u, v, w, h, word, slot, vbase, lo, hi);
+	uint32_t epos = 0;
+	if (mj == 0 && mk > 0) { // emission position, only needed for a split: both passes' starts before it
+		const uint64_t *o64 = reinterpret_cast<const uint64_t *>(s.eh[t]);
+		const uint64_t *t64 = reinterpret_cast<const uint64_t *>(s.tp[t]);
+		epos = s.epre[t] + __popc(starts & below);
+		for (int q = 0; q < 4; ++q) {
+			const int nb = 16 * (u - 4 * q);
+			const uint64_t m = nb >= 64 ? ~0ull : (nb > 0 ? (1ull << nb) - 1ull : 0ull);
+			epos += __popcll(o64[q] & m) + __popcll(t64[q] & m);
+		}
 	}
+	quad_done(s, tr, mk, mj, epos, lo, hi, box);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -1018,7 +1056,7 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 		uint32_t *dst = s.vox + row * kHalo;
 #pragma unroll
 		for (int x = 0; x < 18; ++x) {
-			dst[x] = lt[x] | pv[x];
+			dst[x] = vox_light(lt[x]) | pv[x];
 			mixed |= (pv[x] ^ kVoxOpaqueCube) & kVoxShape;
 			if (x >= 1 && x <= 16)
 				inner |= pv[x] & kVoxShape;
@@ -1052,10 +1090,13 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 		if (s.wave_ref[w] != 0xffffffffu && s.wave_ref[w] != s.wave_ref[0])
 			flags |= kFlagNonUniform;
 
-	if (!(flags & kFlagCustom) &&
-	    (!(flags & kFlagNonUniform) || !(flags & kFlagMixed) || !(flags & kFlagInnerFaces))) {
-		// one block value everywhere (every face is hidden by an equal neighbour), opaque cubes only, or
-		// no chunk voxel with a face: no quads
+	// one block value everywhere hides every face when the block's opposite textures do not show to each
+	// other (true of every registered block; a registry may define one whose do)
+	const bool uniform_hidden =
+	    !(flags & kFlagNonUniform) && !(s.rows[block_pid(p, s, s.wave_ref[0])].y & kRowSelfShow);
+	if (!(flags & kFlagCustom) && (uniform_hidden || !(flags & kFlagMixed) || !(flags & kFlagInnerFaces))) {
+		// one block value everywhere whose faces hide each other, opaque cubes only, or no chunk voxel
+		// with a face: no quads
 		if (tid == 0)
 			p.results[job] = hcm_job_result{HCM_OK, 0u, 0u, 0u};
 		return;
@@ -1063,12 +1104,12 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 
 	// 2. liquid-flow cells take the flow mesh's face textures (:221-236)
 	if (flags & kFlagFlow) {
-		constexpr uint32_t kKeep = 0xffu | kVoxDirect | kVoxIndirect;
+		constexpr uint32_t kKeep = kVoxSunTorch | kVoxDirect | kVoxIndirect;
 		for (int i = tid; i < kHaloVox; i += NT) {
 			if (!(s.vox[i] & kVoxFlow)) // each thread rewrites only its own cells
 				continue;
 			const int hx = i % kHalo, rest = i / kHalo, hz = rest % kHalo, hy = rest / kHalo;
-			const uint32_t pid = s.vox[i] >> kVoxPidShift;
+			const uint32_t pid = vox_pid(s.vox[i]);
 			const uint32_t mesh = s.rows[pid].y >> 24;
 			const Flow f = flow_heights(p, s.slots, hx - 1, hy - 1, hz - 1, p.reg.meshes[mesh].flow_height);
 			const uint32_t tmpl = s.pid_vox[p.reg.flow_pid_base + flow_pattern(f)];
@@ -1105,34 +1146,25 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 		}
 		const uint32_t tot_a = __shfl(ia, 63);
 		if (lane < kTasks)
-			s.tpre[lane] = ia - a;
+			s.fpre[lane] = ia - a;
 		if (lane + 64 < kTasks)
-			s.tpre[lane + 64] = tot_a + ib - b;
+			s.fpre[lane + 64] = tot_a + ib - b;
 		if (lane == 63)
-			s.tpre[kTasks] = tot_a + ib;
+			s.fpre[kTasks] = tot_a + ib;
 	}
 	__syncthreads();
-	if (s.tpre[kTasks] == 0 && !(flags & kFlagCustom)) { // no visible face and no custom mesh: no quads
+	if (s.fpre[kTasks] == 0 && !(flags & kFlagCustom)) { // no visible face and no custom mesh: no quads
 		if (tid == 0)
 			p.results[job] = hcm_job_result{HCM_OK, 0u, 0u, 0u};
 		return;
 	}
+	// face words per group of tasks, then the group's link masks; the last group's words stay in key
+	constexpr int kMine = (kListCap + NT - 1) / NT;
+	int t0_last = 0;
 #pragma unroll 1
 	for (int t0 = 0; t0 < kTasks;) {
-		if (tid == 0) { // the longest run of tasks whose faces fit (one task, at most 256 faces, always fits)
-			int lo = t0 + 1, hi = kTasks; // largest t1 with tpre[t1] - tpre[t0] <= kListCap
-			while (lo < hi) {
-				const int mid = (lo + hi + 1) >> 1;
-				if (s.tpre[mid] - s.tpre[t0] <= (uint32_t)kListCap)
-					lo = mid;
-				else
-					hi = mid - 1;
-			}
-			s.group_end = lo;
-		}
-		__syncthreads();
-		const int t1 = s.group_end;
-		const uint32_t g0 = s.tpre[t0], n = s.tpre[t1] - g0;
+		const int t1 = group_end(s, t0);
+		const uint32_t g0 = s.fpre[t0], n = s.fpre[t1] - g0;
 		for (int k = tid; k < (t1 - t0) * 8; k += NT) { // clear the group's link masks
 			const int t = t0 + (k >> 3), q = k & 7;
 			reinterpret_cast<uint32_t *>(s.eh[t])[q] = 0;
@@ -1141,7 +1173,6 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 		}
 		// a lane keeps the positions of its faces (at most kListCap / NT of them) for the links, in a
 		// register queue shifted once per step (loops unrolled by two, static register indices)
-		constexpr int kMine = (kListCap + NT - 1) / NT;
 		uint32_t mine[kMine] = {};
 #pragma unroll 2
 		for (int k = 0; k < kMine; ++k) {
@@ -1165,183 +1196,111 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 				mine[j] = mine[j + 1];
 		}
 		__syncthreads();
+		t0_last = t0;
 		t0 = t1;
 	}
 
-	// 4-6. one rolled loop (one copy of the greedy merge and of the custom-cell walk in the code):
-	// pass 0 counts the quads -- custom faces per thread range of cells, greedy quads per task -- and
-	// allocates the chunk's output: [pool_base, +n0) opaque, [+n0, +n0+n1) transparent. Each later
-	// pass describes the quads of one window of the chunk's emission order into the LDS list and
-	// writes them, one lane per quad.
+	// 4. counting: custom faces per thread range of cells, greedy quads per task (one lane per task)
 	uint32_t c_o = 0, c_t = 0, q_o = 0, q_t = 0;
-	uint64_t cust_pre = 0, task_pre = 0;
-	uint32_t n0 = 0, n1 = 0, total = 0, pool_base = 0;
-	// AABB<uint32_t>::Merge (dep/AABB.hpp:52-59) of the first mesh of each pass, per lane, reduced at the end
+	if (flags & kFlagCustom) {
+		for (int k = 0; k < CPT; ++k) {
+			const int cell = tid * CPT + k;
+			if (!((s.custom_bits[cell >> 5] >> (cell & 31)) & 1u))
+				continue;
+			const CustomCell c = custom_cell(p, s, cell);
+			const uint32_t nt = __popc(c.trans);
+			c_t += nt;
+			c_o += c.count - nt;
+		}
+	}
+	if (wave * 64 < kTasks)
+		greedy_count(s, tid, tid < kTasks, q_o, q_t);
+	// 5. block-wide prefix sums and allocation: the chunk's output is [pool_base, +n0) opaque and
+	// [+n0, +n0+n1) transparent; custom quads come first in each pass, then the tasks' quads in task order
+	uint64_t cust_total, task_total;
+	const uint64_t cust_pre = block_excl_scan(s, (uint64_t)c_o | ((uint64_t)c_t << 32), &cust_total);
+	const uint64_t task_pre = block_excl_scan(s, (uint64_t)q_o | ((uint64_t)q_t << 32), &task_total);
+	const uint32_t cust0 = (uint32_t)cust_total, cust1 = (uint32_t)(cust_total >> 32);
+	const uint32_t n0 = cust0 + (uint32_t)task_total, n1 = cust1 + (uint32_t)(task_total >> 32);
+	const uint32_t total = n0 + n1;
+	if (total == 0) {
+		if (tid == 0)
+			p.results[job] = hcm_job_result{HCM_OK, 0u, 0u, 0u};
+		return;
+	}
+	if (tid < kTasks) {
+		s.qpre_o[tid] = cust0 + (uint32_t)task_pre;
+		s.qpre_t[tid] = cust1 + (uint32_t)(task_pre >> 32);
+		s.epre[tid] = cust0 + cust1 + (uint32_t)task_pre + (uint32_t)(task_pre >> 32);
+	}
+	if (tid == 0) {
+		const uint32_t nm0 = (n0 + kSplitQuads - 1) / kSplitQuads, nm1 = (n1 + kSplitQuads - 1) / kSplitQuads;
+		uint32_t status = HCM_OK;
+		const uint32_t pbase = atomicAdd(&p.ctr->pool_cursor, total);
+		if ((uint64_t)pbase + total > p.pool_cap)
+			status = (uint32_t)HCM_ERR_POOL_FULL;
+		const uint32_t mbase = atomicAdd(&p.ctr->mesh_cursor, nm0 + nm1);
+		if ((uint64_t)mbase + nm0 + nm1 > p.rec_cap)
+			status = (uint32_t)HCM_ERR_POOL_FULL;
+		s.status = status;
+		s.pool_base = pbase;
+		s.mesh_base = mbase;
+		s.n_mesh0 = nm0;
+		s.n_mesh1 = nm1;
+		if (status != HCM_OK)
+			p.results[job] = hcm_job_result{(int32_t)status, 0u, 0u, total};
+	}
+	__syncthreads();
+	if (s.status != HCM_OK)
+		return;
+	const uint32_t pool_base = s.pool_base;
+
+	// 6. emission. AABB<uint32_t>::Merge (dep/AABB.hpp:52-59) of the first mesh of each pass per lane,
+	// reduced at the end; later meshes of a pass (past 16,383 quads) by LDS atomics
 	uint32_t box[2][6];
 #pragma unroll
 	for (int a = 0; a < 6; ++a)
 		box[0][a] = box[1][a] = a < 3 ? ~0u : 0u;
+	// custom quads, in the reference's order (cells in YZX order, faces in mesh order), per thread range
+	if (flags & kFlagCustom) {
+		uint32_t io = (uint32_t)cust_pre, it = (uint32_t)(cust_pre >> 32);
+		for (int k = 0; k < CPT; ++k) {
+			const int cell = tid * CPT + k;
+			if (!((s.custom_bits[cell >> 5] >> (cell & 31)) & 1u))
+				continue;
+			const CustomCell c = custom_cell(p, s, cell);
+			for (uint32_t j = 0; j < c.count; ++j) {
+				const uint32_t tr = (c.trans >> j) & 1u, epos = io + it, i = tr ? it++ : io++;
+				uint32_t slot, vbase, mk, mj, lo[3], hi[3];
+				quad_slot(tr, i, n0, pool_base, slot, vbase, mk, mj);
+				emit_custom(p, s, uint32_t(cell) | (j << 12), slot, vbase, lo, hi);
+				quad_done(s, tr, mk, mj, epos, lo, hi, box);
+			}
+		}
+	}
+	// regular quads, one lane per visible face (a quad per face that starts one): first the last group,
+	// whose face words are still in key, then the earlier groups with their words computed again
+	{
+		const uint32_t g0 = s.fpre[t0_last], n = s.fpre[kTasks] - g0;
 #pragma unroll 1
-	for (uint32_t gp = 0, w0 = 0;; ++gp) {
-		const bool counting = gp == 0;
-		const uint32_t w1 = w0 + kListCap;
-		if (flags & kFlagCustom) {
-			uint32_t io = (uint32_t)cust_pre, it = (uint32_t)(cust_pre >> 32), g = io + it;
-			if (counting || g < w1) {
-				for (int k = 0; k < CPT; ++k) {
-					const int cell = tid * CPT + k;
-					if (!((s.custom_bits[cell >> 5] >> (cell & 31)) & 1u))
-						continue;
-					const CustomCell c = custom_cell(p, s, cell);
-					if (counting) {
-						const uint32_t nt = __popc(c.trans);
-						c_t += nt;
-						c_o += c.count - nt;
-						continue;
-					}
-					for (uint32_t j = 0; j < c.count; ++j, ++g) {
-						const uint32_t tr = (c.trans >> j) & 1u, i = tr ? it++ : io++;
-						if (g >= w0 && g < w1)
-							s.list[g - w0] = make_uint2(0x80000000u | uint32_t(cell) | (j << 12), i | (tr << 31));
-					}
-				}
-			}
-		}
-		if (counting) {
-			if (wave * 64 < kTasks)
-				greedy_count(s, tid, tid < kTasks, q_o, q_t);
-		} else {
-			const uint32_t n_rows = s.rpre[kTasks];
+		for (uint32_t i = tid; i < n; i += NT)
+			emit_face(p, s, t0_last, kTasks, g0 + i, s.key[i], n0, pool_base, box);
+	}
 #pragma unroll 1
-			for (uint32_t i = tid; i < n_rows; i += NT) { // the mask rows holding quad starts
-				int lo = 0, hi = kTasks - 1; // task: rpre[t] <= i < rpre[t + 1]
-				while (lo < hi) {
-					const int mid = (lo + hi + 1) >> 1;
-					if (s.rpre[mid] <= i)
-						lo = mid;
-					else
-						hi = mid - 1;
-				}
-				const int t = lo;
-				const uint32_t tg = s.tpre[t];
-				if (tg >= w1 || s.tpre[t + 1] <= w0) // none of the task's quads is in this window
-					continue;
-				uint32_t bits = s.rmask[t], rem = i - s.rpre[t];
-				while (rem--)
-					bits &= bits - 1;
-				const int u = __ffs(bits) - 1;
-				const uint32_t to = s.qpre_o[t];
-				describe_row(s, t, u, [&](const Quad &q) {
-					const uint32_t g = tg + q.ord;
-					if (g >= w0 && g < w1) {
-						const uint32_t i = q.trans ? (tg - to) + q.ord_pass : to + q.ord_pass;
-						s.list[g - w0] = make_uint2(uint32_t(t) | (q.u << 7) | (q.v0 << 11) | ((q.w - 1) << 15) |
-						                                ((q.h - 1) << 19),
-						                            i | (q.trans << 31));
-					}
-				});
-			}
-		}
-		if (counting) {
-			uint64_t cust_total, task_total;
-			cust_pre = block_excl_scan(s, (uint64_t)c_o | ((uint64_t)c_t << 32), &cust_total);
-			task_pre = block_excl_scan(s, (uint64_t)q_o | ((uint64_t)q_t << 32), &task_total);
-			const uint32_t cust0 = (uint32_t)cust_total, cust1 = (uint32_t)(cust_total >> 32);
-			n0 = cust0 + (uint32_t)task_total, n1 = cust1 + (uint32_t)(task_total >> 32);
-			total = n0 + n1;
-			if (total == 0) {
-				if (tid == 0)
-					p.results[job] = hcm_job_result{HCM_OK, 0u, 0u, 0u};
-				return;
-			}
-			// per task: first emission position (custom quads come first) and first opaque index; the
-			// first transparent index is the difference of the two
-			if (tid < kTasks) {
-				s.tpre[tid] = cust0 + cust1 + (uint32_t)task_pre + (uint32_t)(task_pre >> 32);
-				s.qpre_o[tid] = cust0 + (uint32_t)task_pre;
-			}
-			if (tid == 0)
-				s.tpre[kTasks] = total;
-			if (wave == 0) { // exclusive prefix over the tasks of their rows holding quad starts
-				const uint32_t a = lane < kTasks ? __popc(s.rmask[lane]) : 0u;
-				const uint32_t b = lane + 64 < kTasks ? __popc(s.rmask[lane + 64]) : 0u;
-				uint32_t ia = a, ib = b;
-#pragma unroll
-				for (int d = 1; d < 64; d <<= 1) {
-					const uint32_t xa = __shfl_up(ia, d), xb = __shfl_up(ib, d);
-					if (lane >= d)
-						ia += xa, ib += xb;
-				}
-				const uint32_t tot_a = __shfl(ia, 63);
-				if (lane < kTasks)
-					s.rpre[lane] = ia - a;
-				if (lane + 64 < kTasks)
-					s.rpre[lane + 64] = tot_a + ib - b;
-				if (lane == 63)
-					s.rpre[kTasks] = tot_a + ib;
-			}
-			// 5. allocation
-			if (tid == 0) {
-				const uint32_t nm0 = (n0 + kSplitQuads - 1) / kSplitQuads, nm1 = (n1 + kSplitQuads - 1) / kSplitQuads;
-				uint32_t status = HCM_OK;
-				const uint32_t pbase = atomicAdd(&p.ctr->pool_cursor, total);
-				if ((uint64_t)pbase + total > p.pool_cap)
-					status = (uint32_t)HCM_ERR_POOL_FULL;
-				const uint32_t mbase = atomicAdd(&p.ctr->mesh_cursor, nm0 + nm1);
-				if ((uint64_t)mbase + nm0 + nm1 > p.rec_cap)
-					status = (uint32_t)HCM_ERR_POOL_FULL;
-				s.status = status;
-				s.pool_base = pbase;
-				s.mesh_base = mbase;
-				s.n_mesh0 = nm0;
-				s.n_mesh1 = nm1;
-				if (status != HCM_OK)
-					p.results[job] = hcm_job_result{(int32_t)status, 0u, 0u, total};
-			}
-			__syncthreads();
-			if (s.status != HCM_OK)
-				return;
-			pool_base = s.pool_base;
-			continue;
-		}
-		// 6. write the window's quads
-		__syncthreads();
-		const uint32_t n = min(total - w0, (uint32_t)kListCap);
-		for (uint32_t e = tid; e < n; e += NT) {
-			const uint2 ent = s.list[e];
-			const uint32_t pass = ent.y >> 31, i = ent.y & 0x7fffffffu;
-			const uint32_t slot = pool_base + (pass ? n0 : 0u) + i;
-			const uint32_t mk = i / kSplitQuads, mj = i - mk * kSplitQuads; // mesh of the pass, index in it
-			// the reference reads the 16-bit vertex count before splitting, so the first quad of a
-			// continuation mesh keeps base 65,532 (:273-279, :397-403)
-			const uint32_t vbase = (mj == 0 && mk > 0) ? 65532u : 4u * mj;
-			uint32_t lo[3], hi[3];
-			if (ent.x & 0x80000000u)
-				emit_custom(p, s, ent.x, slot, vbase, lo, hi);
-			else
-				emit_regular(p, s, ent.x, slot, vbase, lo, hi);
-			if (mk == 0) { // the pass's first mesh: this lane's running box (static indices only)
-#pragma unroll
-				for (int a = 0; a < 3; ++a) {
-					box[0][a] = pass ? box[0][a] : min(box[0][a], lo[a]);
-					box[0][3 + a] = pass ? box[0][3 + a] : max(box[0][3 + a], hi[a]);
-					box[1][a] = pass ? min(box[1][a], lo[a]) : box[1][a];
-					box[1][3 + a] = pass ? max(box[1][3 + a], hi[a]) : box[1][3 + a];
-				}
-			} else
-				aabb_merge(s, pass, mk, lo, hi);
-			if (mj == 0 && mk > 0) { // a split happened right before this quad (:397-403)
-				const uint32_t ev = atomicAdd(&s.n_events, 1u);
-				if (ev < 16) {
-					s.events[ev][0] = w0 + e;
-					s.events[ev][1] = (pass << 16) | (mk - 1u);
-				}
-			}
+	for (int t0 = 0; t0 < t0_last;) {
+		__syncthreads(); // key is read by the previous group's lanes
+		const int t1 = group_end(s, t0);
+		const uint32_t g0 = s.fpre[t0], n = s.fpre[t1] - g0;
+#pragma unroll 1
+		for (uint32_t i = tid; i < n; i += NT) {
+			uint32_t d;
+			s.key[i] = face_word(s, t0, t1, g0 + i, d);
 		}
 		__syncthreads();
-		w0 += kListCap;
-		if (w0 >= total)
-			break;
+#pragma unroll 1
+		for (uint32_t i = tid; i < n; i += NT)
+			emit_face(p, s, t0, t1, g0 + i, s.key[i], n0, pool_base, box);
+		t0 = t1;
 	}
 
 #pragma unroll
@@ -1492,6 +1451,28 @@ hipError_t launch_edits(uint16_t *blocks, uint8_t *light, int8_t *heights, const
 	if (n == 0)
 		return hipSuccess;
 	hipLaunchKernelGGL(edit_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, blocks, light, heights, edits, n);
+	return hipGetLastError();
+}
+
+// Scatters a packed chunk halo (include/hcm.h, HCM_HALO_BLOCKS / HCM_HALO_LIGHT) into a 27-slot store in
+// neighbour-slot order, where the mesher reads it; cells of the slots outside the halo are not written
+// (the mesher never reads them).
+__global__ void halo_unpack_kernel(const uint16_t *hblocks, const uint8_t *hlight, uint16_t *blocks, uint8_t *light) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < HCM_HALO_BLOCKS) {
+		const int x = i % 20 - 2, z = (i / 20) % 20 - 2, y = i / 400 - 1;
+		blocks[(size_t)slot27(x, y, z) * HCM_CHUNK_VOXELS + local16(x, y, z)] = hblocks[i];
+	}
+	if (i < HCM_HALO_LIGHT) {
+		const int x = i % 18 - 1, z = (i / 18) % 18 - 1, y = i / 324 - 1;
+		light[(size_t)slot27(x, y, z) * HCM_CHUNK_VOXELS + local16(x, y, z)] = hlight[i];
+	}
+}
+
+hipError_t launch_halo_unpack(const uint16_t *hblocks, const uint8_t *hlight, uint16_t *blocks, uint8_t *light,
+                              hipStream_t stream) {
+	hipLaunchKernelGGL(halo_unpack_kernel, dim3((HCM_HALO_BLOCKS + 255) / 256), dim3(256), 0, stream, hblocks, hlight,
+	                   blocks, light);
 	return hipGetLastError();
 }
 

@@ -22,6 +22,9 @@ HCM_ERR_CAPACITY = -4
 HCM_ERR_NO_DEVICE = -5
 HCM_ERR_BUSY = -6
 
+HALO_BLOCKS = 20 * 20 * 18  # blocks over x, z in [-2, 18), y in [-1, 17): index (x+2) + 20 ((z+2) + 20 (y+1))
+HALO_LIGHT = 18 * 18 * 18   # light over [-1, 17)^3: index (x+1) + 18 ((z+1) + 18 (y+1))
+
 RUN_PERSISTENT = 0
 RUN_PER_CHUNK = 1
 LIGHT_STORE = 0    # mesh with the store's light bytes
@@ -116,6 +119,7 @@ SIGNATURES = [
     ("hcm_last_hip_error", ctypes.c_int, []),
     ("hcm_device_ok", ctypes.c_int, []),
     ("hcm_mesh_chunk", ctypes.c_int, [_VP, _VP, _VP, ctypes.POINTER(HostResultC)]),
+    ("hcm_mesh_halo", ctypes.c_int, [_VP, _VP, _VP, ctypes.POINTER(HostResultC)]),
     ("hcm_mesh_chunk_relight", ctypes.c_int, [_VP, _VP, _VP, ctypes.POINTER(HostResultC), _VP]),
     ("hcm_store_create", ctypes.c_int, [_VP, ctypes.c_uint32, ctypes.POINTER(_VP)]),
     ("hcm_store_destroy", None, [_VP]),
@@ -298,6 +302,13 @@ class Context:
         pb, keep_b = _ptrs27(blocks27, np.uint16)
         pl, keep_l = _ptrs27(light27, np.uint8)
         check("hcm_mesh_chunk", lib().hcm_mesh_chunk(self._ptr, pb, pl, ctypes.byref(self._out.c)))
+        return self._out.meshes()
+
+    def mesh_halo(self, halo_blocks, halo_light):
+        """Plain path from a packed halo (halo_of_chunks): the drop-in call's 20 KB upload."""
+        b = np.ascontiguousarray(halo_blocks, np.uint16).reshape(HALO_BLOCKS)
+        l = np.ascontiguousarray(halo_light, np.uint8).reshape(HALO_LIGHT)
+        check("hcm_mesh_halo", lib().hcm_mesh_halo(self._ptr, b.ctypes.data, l.ctypes.data, ctypes.byref(self._out.c)))
         return self._out.meshes()
 
     def mesh_chunk_relight(self, blocks27, heights27):
@@ -517,6 +528,31 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def halo_of_chunks(blocks27, light27):
+    """The packed halo (include/hcm.h, HCM_HALO_BLOCKS / HCM_HALO_LIGHT) of the chunk in slot 26 of 27
+    chunks in neighbour-slot order: (blocks (18, 20, 20) u16 [y, z, x], light (18, 18, 18) u8 [y, z, x])."""
+    lut = {-1: 1, 0: 2, 1: 0}
+    b = np.zeros((18, 20, 20), np.uint16)
+    l = np.zeros((18, 18, 18), np.uint8)
+    for cy in (-1, 0, 1):
+        for cz in (-1, 0, 1):
+            for cx in (-1, 0, 1):
+                slot = 9 * lut[cx] + 3 * lut[cy] + lut[cz]
+                cb = np.asarray(blocks27[slot], np.uint16).reshape(16, 16, 16)
+                cl = np.asarray(light27[slot], np.uint8).reshape(16, 16, 16)
+                # world coordinate w of local c in this chunk: w = 16 * chunk + c
+                def span(c, lo, hi):
+                    w0, w1 = max(lo, 16 * c), min(hi, 16 * c + 16)
+                    return (slice(w0 - lo, w1 - lo), slice(w0 - 16 * c, w1 - 16 * c)) if w0 < w1 else None
+                sy, sz, sx = span(cy, -1, 17), span(cz, -2, 18), span(cx, -2, 18)
+                if sy and sz and sx:
+                    b[sy[0], sz[0], sx[0]] = cb[sy[1], sz[1], sx[1]]
+                sz, sx = span(cz, -1, 17), span(cx, -1, 17)
+                if sy and sz and sx:
+                    l[sy[0], sz[0], sx[0]] = cl[sy[1], sz[1], sx[1]]
+    return b, l
 
 
 def job_meshes(job, res, recs, verts, idx):

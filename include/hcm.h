@@ -133,10 +133,25 @@ typedef struct hcm_host_result {
 } hcm_host_result;
 
 /* Meshes the chunk in slot 26 of blocks/light (27 host arrays of 4096 voxels each), like
- * Generate(get_block, get_light) over the 27-chunk lambdas of ChunkMeshTask.cpp:117-122.
+ * Generate(get_block, get_light) over the 27-chunk lambdas of ChunkMeshTask.cpp:117-122: gathers the
+ * chunk's packed halo from them on the host and meshes it as hcm_mesh_halo does.
  * Thread-safe per context (calls are serialised on the context's stream). */
 int hcm_mesh_chunk(hcm_ctx *ctx, const uint16_t *const blocks[27], const uint8_t *const light[27],
                    hcm_host_result *out);
+
+/* Packed halo of one chunk: exactly the voxels the mesher reads around the 16^3 chunk. Blocks over
+ * x, z in [-2, 18) and y in [-1, 17) -- the 18^3 halo plus the x/z ring that flowing water reads around
+ * halo cells (BlockMeshAlgo.hpp:202-236) -- at index (x+2) + 20 ((z+2) + 20 (y+1)); light over the 18^3
+ * halo [-1, 17)^3 at index (x+1) + 18 ((z+1) + 18 (y+1)). 14,400 + 5,832 bytes per chunk. */
+#define HCM_HALO_BLOCKS (20 * 20 * 18)
+#define HCM_HALO_LIGHT (18 * 18 * 18)
+
+/* Meshes one chunk from its packed halo, like Generate(get_block, get_light)
+ * (client/include/client/BlockMeshAlgo.hpp:494-521) at the reference's call site
+ * (client/src/ChunkMeshTask.cpp:113-122), whose lambdas are evaluated exactly over these voxels. The
+ * halo (20 KB) is the only upload; the results of a chunk of up to 2,048 quads come back with one
+ * synchronisation. Same output and errors as hcm_mesh_chunk. */
+int hcm_mesh_halo(hcm_ctx *ctx, const uint16_t *blocks, const uint8_t *light, hcm_host_result *out);
 
 /* The whole mesh task of ChunkTaskRunner<kMesh>::Run (client/src/ChunkMeshTask.cpp:41-122): rebuilds light
  * from the 27 chunks' blocks and sunlight heights (int8[256] each, Chunk::m_sunlight_heights, index

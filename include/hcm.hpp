@@ -6,8 +6,8 @@
 // packed vertices, u16 indices, AABB<uint32_t> in 1/16 block units, and the transparent flag, in the
 // reference's mesh order. The callables are evaluated only where the mesher reads: blocks over
 // x, z in [-2, 18), y in [-1, 17) (the 18^3 halo plus the x/z cells flowing water reads around halo
-// cells, BlockMeshAlgo.hpp:202-236), light over the 18^3 halo [-1, 17)^3; the meshing itself runs on
-// the GPU through include/hcm.h.
+// cells, BlockMeshAlgo.hpp:202-236), light over the 18^3 halo [-1, 17)^3 -- the packed halo of
+// hcm_mesh_halo; the meshing itself runs on the GPU through include/hcm.h.
 #pragma once
 
 #include <array>
@@ -48,10 +48,6 @@ public:
 		int r = hcm_create(registry, pool_quads, &m_ctx);
 		if (r != HCM_OK)
 			throw Error("hcm_create", r);
-		for (int i = 0; i < 27; ++i) {
-			m_block_ptrs[i] = m_blocks[i].data();
-			m_light_ptrs[i] = m_light[i].data();
-		}
 	}
 	~BlockMeshGPU() { hcm_destroy(m_ctx); }
 	BlockMeshGPU(const BlockMeshGPU &) = delete;
@@ -59,18 +55,19 @@ public:
 
 	// Same call shape as BlockMeshAlgo::Generate: get_block(x, y, z) returns something convertible to the
 	// 16-bit Block::m_data (e.g. via .GetData()) and get_light(x, y, z) the 8-bit Light::m_data.
+	// The callables fill the chunk's packed halo (include/hcm.h), the only data sent to the device.
 	template <typename GetBlockFunc, typename GetLightFunc>
 	std::vector<BlockMesh> Generate(GetBlockFunc &&get_block_func, GetLightFunc &&get_light_func) {
 		for (int y = -1; y < 17; ++y)
 			for (int z = -2; z < 18; ++z)
 				for (int x = -2; x < 18; ++x) {
-					const int slot = slot_of(x, y, z);
-					const int local = (x & 15) + 16 * ((z & 15) + 16 * (y & 15));
-					m_blocks[slot][local] = (uint16_t)data_of(get_block_func(x, y, z));
-					const bool halo = x >= -1 && x < 17 && z >= -1 && z < 17;
-					m_light[slot][local] = halo ? (uint8_t)data_of(get_light_func(x, y, z)) : 0;
+					m_halo_blocks[(x + 2) + 20 * ((z + 2) + 20 * (y + 1))] = (uint16_t)data_of(get_block_func(x, y, z));
+					if (x >= -1 && x < 17 && z >= -1 && z < 17)
+						m_halo_light[(x + 1) + 18 * ((z + 1) + 18 * (y + 1))] = (uint8_t)data_of(get_light_func(x, y, z));
 				}
-		return GenerateChunks(m_block_ptrs, m_light_ptrs);
+		return Collect("hcm_mesh_halo", [&](hcm_host_result *out) {
+			return hcm_mesh_halo(m_ctx, m_halo_blocks.data(), m_halo_light.data(), out);
+		});
 	}
 
 	// The 27 chunks directly, in neighbour-slot order (common/include/common/Position.hpp:138-145);
@@ -125,18 +122,12 @@ private:
 		return meshes;
 	}
 
-	static int slot_of(int x, int y, int z) {
-		auto l = [](int v) { return v < 0 ? 1 : (v >= 16 ? 0 : 2); };
-		return 9 * l(x) + 3 * l(y) + l(z);
-	}
 	template <typename T> static auto data_of(const T &v) -> decltype(v.GetData()) { return v.GetData(); }
 	static uint32_t data_of(uint32_t v) { return v; }
 
 	hcm_ctx *m_ctx = nullptr;
-	std::array<std::array<uint16_t, HCM_CHUNK_VOXELS>, 27> m_blocks{};
-	std::array<std::array<uint8_t, HCM_CHUNK_VOXELS>, 27> m_light{};
-	const uint16_t *m_block_ptrs[27];
-	const uint8_t *m_light_ptrs[27];
+	std::array<uint16_t, HCM_HALO_BLOCKS> m_halo_blocks{};
+	std::array<uint8_t, HCM_HALO_LIGHT> m_halo_light{};
 	std::vector<BlockVertex> m_verts = std::vector<BlockVertex>(1 << 16);
 	std::vector<uint16_t> m_idx = std::vector<uint16_t>(3 << 15);
 	std::vector<hcm_mesh_desc> m_desc = std::vector<hcm_mesh_desc>(8);

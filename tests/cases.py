@@ -175,3 +175,60 @@ def faceless(seed=11):
     light = rng.integers(0, 256, blocks.shape).astype(np.uint8)
     nbr = np.arange(27 * len(jobs), dtype=np.uint32).reshape(len(jobs), 27)
     return [n for n, _, _ in jobs], blocks, light, nbr, np.array([w for _, _, w in jobs])
+
+
+# ---- edited registries ----
+class RegistryFile:
+    """The committed registry.bin (the reference's table) parsed into arrays that can be edited, written
+    back for the oracle and handed to hcm_create as an hcm_registry."""
+
+    ROW = np.dtype([("tex", "<u2", 6), ("transparency", "u1"), ("collision", "u1"), ("mesh", "<u2")])
+
+    def __init__(self, path):
+        d = open(path, "rb").read()
+        assert d[:8] == b"HCREG01\x00"
+        n_rows, n_meshes, n_tex, self.hdr3 = np.frombuffer(d[8:24], np.uint32)
+        o = 24
+        self.row_of = np.frombuffer(d[o:o + 131072], np.uint16).copy()
+        o += 131072
+        self.rows = np.frombuffer(d[o:o + 16 * n_rows], self.ROW).copy()
+        o += 16 * n_rows
+        self.meshes = np.frombuffer(d[o:o + 400 * n_meshes], np.uint8).reshape(n_meshes, 400).copy()
+        o += 400 * n_meshes
+        self.tex_flags = np.frombuffer(d[o:o + n_tex], np.uint8).copy()
+        o += n_tex
+        self.liquid = (d[o], d[o + 1], int(np.frombuffer(d[o + 2:o + 4], np.uint16)[0]))
+
+    def write(self, path):
+        hdr = np.array([len(self.rows), len(self.meshes), len(self.tex_flags), self.hdr3], np.uint32)
+        with open(path, "wb") as f:
+            f.write(b"HCREG01\x00" + hdr.tobytes() + self.row_of.tobytes() + self.rows.tobytes() +
+                    self.meshes.tobytes() + self.tex_flags.tobytes() + bytes(self.liquid[:2]) +
+                    np.uint16(self.liquid[2]).tobytes())
+
+    def c_registry(self, hcm):
+        """An hcm_registry over these arrays (kept alive by this object), light levels from the default."""
+        import ctypes
+
+        base = hcm.lib().hcm_registry_default().contents
+        reg = hcm.RegistryC()
+        reg.n_rows, reg.n_meshes, reg.n_tex = len(self.rows), len(self.meshes), len(self.tex_flags)
+        reg.row_of = self.row_of.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16))
+        reg.rows = self.rows.ctypes.data_as(ctypes.POINTER(hcm.BlockRowC))
+        reg.meshes = self.meshes.ctypes.data_as(ctypes.POINTER(hcm.CustomMeshC))
+        reg.tex_flags = self.tex_flags.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+        reg.liquid_source_id, reg.liquid_flow_id, reg.liquid_flow_tex = self.liquid
+        reg.light_level = base.light_level
+        self._c = reg
+        return ctypes.pointer(reg)
+
+
+def self_showing_glass(path):
+    """The reference registry with glass's +x texture replaced by the leaves' texture: two glass blocks
+    side by side along x then show faces to each other (two transparent textures that differ), so a halo
+    of glass alone has quads."""
+    reg = RegistryFile(path)
+    glass, leaves = reg.row_of[0x0009], reg.row_of[0x0010]
+    reg.rows["tex"][glass, 0] = reg.rows["tex"][leaves, 0]
+    assert reg.rows["tex"][glass, 0] != reg.rows["tex"][glass, 1]
+    return reg

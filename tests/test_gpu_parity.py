@@ -56,6 +56,18 @@ def test_plain_path_single_chunks(ctx, oracle, golden, name):
     assert meshdigest.digest(got) == meshdigest.digest(want)
 
 
+@pytest.mark.parametrize("name", ["lone_stone", "slab", "uniform_flow", "terrain_chunk", "terrain_chunk_plants",
+                                  "leaf_checkerboard", "palette_sparse_flow", "palette_sparse_custom"])
+def test_plain_path_packed_halo(ctx, golden, name):
+    """hcm_mesh_halo, the drop-in call's 20 KB form: the packed halo gives the reference's meshes (the
+    leaf checkerboard's 24,576 quads take the second round trip)."""
+    blocks, light, nbr = gset(name)
+    for j in range(0, nbr.shape[0], 4):
+        say(f"packed halo: {name} job {j}")
+        got = ctx.mesh_halo(*hcm.halo_of_chunks(*workloads.slice_jobs(blocks, light, nbr, j)))
+        assert meshdigest.digest(got) == golden[name + "_digest"][j].tobytes(), f"{name} job {j}"
+
+
 def test_plain_path_terrain_chunk_full_meshes(ctx, golden):
     blocks, light, nbr = gset("terrain_chunk")
     got = ctx.mesh_chunk(*workloads.slice_jobs(blocks, light, nbr, 0))
@@ -128,22 +140,55 @@ def test_faceless_chunks(ctx, oracle):
         assert (out[0]["quad_count"][j] > 0) == bool(want_faces[j]), name
 
 
+def test_uniform_halo_of_self_showing_block(tmp_path):
+    """A registry in which two neighbouring blocks of one value show faces to each other (glass with the
+    leaves' texture on +x): a halo of that block alone must not take the uniform early exit; plain and
+    batched results equal the oracle's on that registry."""
+    import cpu_oracle
+    from conftest import REGISTRY
+
+    reg = cases.self_showing_glass(REGISTRY)
+    path = str(tmp_path / "registry_glass.bin")
+    reg.write(path)
+    orc = cpu_oracle.Oracle(path)
+    ctx2 = hcm.Context(pool_quads=1 << 20, registry=reg.c_registry(hcm))
+    try:
+        for value in (0x0009, 0x0109, 0x0001):
+            blocks, light, nbr = cases.uniform(value)
+            b27, l27 = workloads.slice_jobs(blocks, light, nbr, 0)
+            want = meshdigest.digest(orc.mesh(b27, l27))
+            say(f"self-showing uniform halo 0x{value:04x}")
+            assert meshdigest.digest(ctx2.mesh_chunk(b27, l27)) == want
+            out = run_batch(ctx2, blocks, light, nbr)
+            assert (out[0]["status"] == 0).all()
+            assert digests_of(out)[0].tobytes() == want
+            assert (out[0]["quad_count"][0] == 32) == ((value & 0xff) == 0x09)
+    finally:
+        ctx2.close()
+
+
 def test_mixed_uneven_batches(ctx, golden):
-    """Uneven batches drawn from one store holding the cave region and the plant terrain: each batch
-    is a captured graph run once and replayed, and per-chunk launches give the same meshes."""
+    """All 200 uneven batches (log-uniform sizes in 1..4096) drawn from one store holding the cave region
+    and the plant terrain: each batch's first run (launched directly) and its captured graph's replay give
+    the reference's meshes; every tenth batch also through per-chunk launches."""
     blocks, light, nbr, n_cave = workloads.mixed_pool()
     want = np.concatenate([golden["cave_region_digest"], golden["terrain_plants_region_digest"]])
     store = hcm.Store(ctx, blocks.shape[0])
     store.upload(blocks, light)
+    chunks = 0
     try:
-        for k, jobs in enumerate(workloads.mixed_batches(nbr.shape[0], n_batches=12)):
-            say(f"mixed batch {k}: {jobs.size} jobs")
+        for k, jobs in enumerate(workloads.mixed_batches(nbr.shape[0])):
+            if k % 20 == 0:
+                say(f"mixed batch {k}: {jobs.size} jobs")
             batch = hcm.Batch(ctx, store, nbr[jobs])
-            for mode in (hcm.RUN_PERSISTENT, hcm.RUN_PERSISTENT, hcm.RUN_PER_CHUNK):
+            modes = (hcm.RUN_PERSISTENT, hcm.RUN_PERSISTENT) + ((hcm.RUN_PER_CHUNK,) if k % 10 == 0 else ())
+            for mode in modes:
                 batch.run(mode)
                 batch.sync()
                 assert_digests(f"mixed batch {k} mode {mode}", digests_of(batch.download()), want[jobs])
             batch.close()
+            chunks += jobs.size
+        assert chunks > 100000
     finally:
         store.close()
 

@@ -86,3 +86,27 @@ def test_cpp_front_end_builds_and_fails_cleanly_without_a_device(tmp_path):
     assert r.returncode in (0, 3), r.stderr
     if r.returncode == 3:
         assert "-5" in r.stderr
+
+
+def test_packed_halo_holds_the_voxels_the_mesher_reads():
+    """hcm.halo_of_chunks gathers exactly HCM_HALO_BLOCKS / HCM_HALO_LIGHT of include/hcm.h: blocks over
+    x, z in [-2, 18), y in [-1, 17), light over [-1, 17)^3, from the 27 chunks in neighbour-slot order."""
+    from hypercraft_amd import hcm
+
+    rng = np.random.default_rng(3)
+    b27 = [rng.integers(0, 65536, 4096).astype(np.uint16) for _ in range(27)]
+    l27 = [rng.integers(0, 256, 4096).astype(np.uint8) for _ in range(27)]
+    hb, hl = hcm.halo_of_chunks(b27, l27)
+    assert hb.size == hcm.HALO_BLOCKS == 7200 and hl.size == hcm.HALO_LIGHT == 5832
+    lut = {-1: 1, 0: 2, 1: 0}
+
+    def at(arrs, x, y, z):
+        c = [(v >= 16) - (v < 0) for v in (x, y, z)]
+        return arrs[9 * lut[c[0]] + 3 * lut[c[1]] + lut[c[2]]][(x & 15) + 16 * ((z & 15) + 16 * (y & 15))]
+
+    for y in range(-1, 17):
+        for z in range(-2, 18):
+            for x in range(-2, 18):
+                assert hb[y + 1, z + 2, x + 2] == at(b27, x, y, z)
+                if -1 <= x < 17 and -1 <= z < 17:
+                    assert hl[y + 1, z + 1, x + 1] == at(l27, x, y, z)

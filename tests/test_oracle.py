@@ -57,6 +57,25 @@ def test_oracle_matches_reference_golden(oracle, golden, name):
         assert got == want[j].tobytes(), f"{name} job {j}"
 
 
+def test_trace_final_world_golden(oracle):
+    """The spiral serving trace's final world (tests/golden/trace.npz) rebuilds to the same inputs, and
+    the oracle meshes every 16th meshable chunk of it as the reference did."""
+    import importlib.util
+
+    spec = importlib.util.spec_from_file_location("make_golden", os.path.join(GOLDEN, "make_golden.py"))
+    mg = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mg)
+    gold = np.load(os.path.join(GOLDEN, "trace.npz"))
+    tr = workloads.spiral_trace()
+    blocks, light = mg.trace_final_world(tr)
+    pick, nbr = mg.trace_meshable(tr)
+    assert meshdigest.input_digest(blocks, light, nbr).encode() == gold["input"].tobytes()
+    assert np.array_equal(tr["coords"][pick], gold["positions"])
+    for j in range(0, pick.size, 16):
+        got = meshdigest.digest(oracle.mesh(*workloads.slice_jobs(blocks, light, nbr, j)))
+        assert got == gold["digest"][j].tobytes(), tuple(gold["positions"][j])
+
+
 def test_oracle_matches_reference_live(oracle, reference):
     # fresh seeds beyond the fixtures, every family
     for fam in cases.FAMILIES:
@@ -118,6 +137,27 @@ def test_lone_stone_block(oracle):
                                                         word0(6, 7, 7, 0)]
     assert m.indices[6 * k:6 * k + 6].tolist() == [4 * k + 1, 4 * k + 2, 4 * k + 3, 4 * k, 4 * k + 1, 4 * k + 3]
     assert m.aabb == (5 * 16, 7 * 16, 6 * 16, 6 * 16, 8 * 16, 7 * 16)
+
+
+def test_uniform_halo_of_self_showing_block(tmp_path):
+    """Glass whose +x texture is the leaves' texture: neighbouring glass blocks show each other their x
+    faces (two transparent textures that differ), so a halo of glass alone has, per x slice, one 16 x 16
+    quad: - faces (glass texture, opaque pass) at x = 0 .. 15 and + faces (leaves texture, transparent
+    pass) at x = 1 .. 16."""
+    import cpu_oracle
+    from conftest import REGISTRY
+
+    path = str(tmp_path / "registry_glass.bin")
+    cases.self_showing_glass(REGISTRY).write(path)
+    meshes = cpu_oracle.Oracle(path).mesh(*workloads.slice_jobs(*cases.uniform(0x0009), 0))
+    assert [(m.transparent, m.vertices.shape[0] // 4) for m in meshes] == [(0, 16), (1, 16)]
+    assert meshes[0].aabb == (0, 0, 0, 15 * 16, 256, 256) and meshes[1].aabb == (16, 0, 0, 256, 256, 256)
+    for m, xs in zip(meshes, (range(0, 16), range(1, 17))):
+        w0 = m.vertices[:, 0].reshape(16, 4)
+        assert ((w0 >> 30) == 0).all()  # x axis
+        assert sorted(set((w0 & 1023).ravel().tolist())) == [16 * x for x in xs]
+        assert (np.sort((w0 >> 10) & 1023, 1) == [0, 0, 256, 256]).all()  # y spans 0 .. 16
+        assert (np.sort((w0 >> 20) & 1023, 1) == [0, 0, 256, 256]).all()  # z spans 0 .. 16
 
 
 def test_full_slab_is_one_quad(oracle):
