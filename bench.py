@@ -8,17 +8,20 @@ one batched meshing pass over all 4096 chunks (HIP-graph replay of counter reset
 kernel) with the autotuned workgroup shape. Output is checked bit-exact against the digests the
 reference mesher produced for the same chunks.
 
-Three more legs go into the same JSON line:
+More legs go into the same JSON line:
   mixed_batches  200 batches of log-uniform size in [1, 4096] drawn from the cave region and a
                  plant-covered terrain region: captured-graph runs vs per-chunk launches, bit-exact;
   serving        a spiral load trace (radius 24 columns x 16 chunks, ~29k terrain chunks, Poisson
                  arrivals at 20k chunks/s) with concurrent single-voxel edits at 2k/s through the
-                 continuous-batching server (16 GB vertex pool): p50/p99 request latency;
+                 continuous-batching server (16 GB vertex pool): p50/p99 request latency; also with the
+                 light rebuild, and with loads arriving ten times as fast;
   relight        the whole mesh task on a lit cave region: light rebuilt from the blocks' light levels
-                 and the chunks' sunlight heights, then meshing, bit-exact with the reference.
+                 and the chunks' sunlight heights, then meshing, bit-exact with the reference;
+  dropin         the one-chunk call from host memory (hcm_mesh_chunk, hcm_mesh_halo): per-call p50/p99
+                 with the PCIe transfers, beside the reference's Generate on one host thread.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-plain] [--no-mixed] [--no-serve]
-                    [--no-relight]
+                    [--no-relight] [--no-dropin]
 """
 import argparse
 import json
@@ -41,16 +44,36 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(blocks, light, nbr, threads, min_seconds=10.0):
-    """The reference mesher (oracle/_ref, if it was built) else the C oracle, on the host cores:
-    whole passes over the batch until at least min_seconds of wall time have run."""
+# The GPU box gives one GPU's job 16 host cores, whatever the machine has.
+CPU_SHARE = 16
+
+
+def host_cpus():
+    """(cores this process may run on, CPU model name)."""
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return len(os.sched_getaffinity(0)), model
+
+
+def cpu_impl():
+    """The reference mesher (oracle/_ref, where it was built) else the C oracle."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import cpu_oracle
 
     if cpu_oracle.Reference.available():
-        impl, kind = cpu_oracle.Reference(), "reference"
-    else:
-        impl, kind = cpu_oracle.Oracle(os.path.join(ROOT, "tests", "golden", "registry.bin")), "port"
+        return cpu_oracle.Reference(), "reference"
+    return cpu_oracle.Oracle(os.path.join(ROOT, "tests", "golden", "registry.bin")), "port"
+
+
+def cpu_passes(impl, blocks, light, nbr, threads, min_seconds):
+    """Whole passes over the batch with `threads` host threads until min_seconds have run: (chunks/s, quads/s,
+    passes, seconds)."""
     passes, quads = 0, 0
     t0 = time.perf_counter()
     while True:
@@ -59,11 +82,80 @@ def cpu_baseline(blocks, light, nbr, threads, min_seconds=10.0):
         dt = time.perf_counter() - t0
         if dt >= min_seconds:
             break
-    chunks = passes * nbr.shape[0]
-    return {"value": round(chunks / dt, 1), "unit": "chunks/s", "cores": threads, "kind": kind,
-            "sample": f"{passes} passes over the {nbr.shape[0]}-chunk batch ({chunks} chunks, {quads} quads), "
-                      f"{dt:.2f} s wall, {threads} threads",
-            "quads_per_s": round(quads / dt, 1)}
+    return passes * nbr.shape[0] / dt, quads / dt, passes, dt
+
+
+def cpu_baseline(blocks, light, nbr, seconds=(8.0, 6.0)):
+    """The reference mesher on the host cores: N threads, N the cores this process may use up to the box's
+    share for one GPU, and the reference's default of 3/4 of them (client/src/Application.cpp:248)."""
+    impl, kind = cpu_impl()
+    host, model = host_cpus()
+    n = min(host, CPU_SHARE)
+    n34 = max(1, n * 3 // 4)
+    v, q, passes, dt = cpu_passes(impl, blocks, light, nbr, n, seconds[0])
+    v34, q34, passes34, dt34 = cpu_passes(impl, blocks, light, nbr, n34, seconds[1])
+    return {"value": round(v, 1), "unit": "chunks/s", "cores": n, "kind": kind,
+            "sample": f"{passes} passes over the {nbr.shape[0]}-chunk batch ({passes * nbr.shape[0]} chunks), "
+                      f"{dt:.2f} s wall, {n} threads; then {passes34} passes with {n34} threads in {dt34:.2f} s",
+            "quads_per_s": round(q, 1), "threads_3_4": n34, "value_3_4": round(v34, 1),
+            "quads_per_s_3_4": round(q34, 1), "host_cores": host, "cpu_model": model,
+            "threads_policy": f"min(cores available = {host}, {CPU_SHARE} per GPU on this box); 3/4 of that is "
+                              "the reference's default worker count"}
+
+
+def dropin_leg(ctx, hcm, blocks, light, nbr, golden_digest, n=512, reps=3):
+    """The drop-in call as the reference's ChunkTaskRunner<kMesh>::Run makes it (ChunkMeshTask.cpp:113-122):
+    one chunk per call from host memory, meshes back in host memory, PCIe both ways. hcm_mesh_chunk takes
+    the 27 chunks (the library packs the 20 KB halo), hcm_mesh_halo a halo packed by the caller; per-call
+    latency over the first n cave jobs. Beside it the reference's Generate on one host thread, same chunks."""
+    import ctypes
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import meshdigest
+
+    lib = hcm.lib()
+    out = hcm._HostOut()
+    jobs = range(min(n, nbr.shape[0]))
+    arrays = [workloads_slice(blocks, light, nbr, j) for j in jobs]
+    ptrs = []
+    for b27, l27 in arrays:
+        pb = (ctypes.c_void_p * 27)(*[x.ctypes.data for x in b27])
+        pl = (ctypes.c_void_p * 27)(*[x.ctypes.data for x in l27])
+        ptrs.append((pb, pl))
+    halos = [tuple(np.ascontiguousarray(a) for a in hcm.halo_of_chunks(b27, l27)) for b27, l27 in arrays]
+    bad = 0
+    for j in range(0, len(ptrs), 64):  # the calls give the reference's meshes
+        hcm.check("hcm_mesh_chunk", lib.hcm_mesh_chunk(ctx.ptr, ptrs[j][0], ptrs[j][1], ctypes.byref(out.c)))
+        bad += meshdigest.digest(out.meshes()) != golden_digest[j].tobytes()
+        hcm.check("hcm_mesh_halo", lib.hcm_mesh_halo(ctx.ptr, halos[j][0].ctypes.data, halos[j][1].ctypes.data,
+                                                     ctypes.byref(out.c)))
+        bad += meshdigest.digest(out.meshes()) != golden_digest[j].tobytes()
+    lat = {"chunk": [], "halo": []}
+    for _ in range(reps):
+        for (pb, pl), (hb, hl) in zip(ptrs, halos):
+            t0 = time.perf_counter()
+            hcm.check("hcm_mesh_chunk", lib.hcm_mesh_chunk(ctx.ptr, pb, pl, ctypes.byref(out.c)))
+            t1 = time.perf_counter()
+            hcm.check("hcm_mesh_halo", lib.hcm_mesh_halo(ctx.ptr, hb.ctypes.data, hl.ctypes.data, ctypes.byref(out.c)))
+            t2 = time.perf_counter()
+            lat["chunk"].append((t1 - t0) * 1e3)
+            lat["halo"].append((t2 - t1) * 1e3)
+    impl, kind = cpu_impl()
+    sub = np.ascontiguousarray(nbr[:len(ptrs)])
+    t0 = time.perf_counter()
+    impl.mesh_many(blocks, light, sub, 1)
+    ref_ms = (time.perf_counter() - t0) * 1e3 / len(ptrs)
+    return {"chunks": len(ptrs), "calls_per_form": len(ptrs) * reps, "bit_exact": bad == 0,
+            "mesh_chunk_p50_ms": percentile(lat["chunk"], 50), "mesh_chunk_p99_ms": percentile(lat["chunk"], 99),
+            "mesh_halo_p50_ms": percentile(lat["halo"], 50), "mesh_halo_p99_ms": percentile(lat["halo"], 99),
+            "upload_bytes_halo": int(hcm.HALO_BLOCKS * 2 + hcm.HALO_LIGHT),
+            "cpu_generate_1_thread_ms": round(ref_ms, 4), "cpu_kind": kind,
+            "note": "wall time per call from Python through ctypes (a few microseconds of call overhead), host "
+                    "buffers in and out"}
+
+
+def workloads_slice(blocks, light, nbr, j):
+    return [blocks[s] for s in nbr[j]], [light[s] for s in nbr[j]]
 
 
 def relight_leg(ctx, hcm, workloads, meshdigest, steps, cpu_threads, with_cpu):
@@ -126,12 +218,11 @@ def relight_leg(ctx, hcm, workloads, meshdigest, steps, cpu_threads, with_cpu):
     return res
 
 
-def draw_leg(hcm, workloads, batch, n_meshes):
+def draw_leg(hcm, workloads, batch, n_meshes, pos):
     """The draw list after meshing: mesh infos of the cave batch's meshes (WorldRenderer::PushChunkMesh),
     frustum culling into indirect draws (chunk_culling.comp) for 20 seeded cameras, and the depth hierarchy
     of a 1920 x 1080 depth image plus occlusion culling for the same cameras; wall time per call, host
     copies of the lists (and of the depth image) included."""
-    pos = workloads.region_job_positions()
     batch.mesh_infos(pos)
     t0 = time.perf_counter()
     for _ in range(5):
@@ -299,6 +390,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--no-relight", action="store_true")
+    ap.add_argument("--no-dropin", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (for counter calibration)")
     args = ap.parse_args()
@@ -321,14 +413,19 @@ def main():
     ctx = hcm.Context(pool_quads=1 << 24)
     store = hcm.Store(ctx, blocks.shape[0])
     store.upload(blocks, light)
-    batch = hcm.Batch(ctx, store, nbr)
     n = nbr.shape[0]
+    positions = workloads.region_job_positions()
 
     tuning = ctx.get_tuning()
+    tuning["order"] = "linear"
+    perm = np.arange(n)
     if not args.no_tune:
         log("autotune")
-        tuned = tune.autotune(ctx, batch, workload="cave_region", reps=3, log=log)
-        tuning = {"threads": tuned["threads"], "wg_per_cu": tuned["wg_per_cu"]}
+        tuned = tune.autotune(ctx, store, nbr, workload="cave_region", positions=positions,
+                              want=golden["cave_region_digest"], reps=3, log=log)
+        tuning = {"threads": tuned["threads"], "wg_per_cu": tuned["wg_per_cu"], "order": tuned["order"]}
+        perm = tuned["perm"]
+    batch = hcm.Batch(ctx, store, nbr[perm])  # job k is the region's job perm[k]
 
     log(f"warmup {args.warmup}")
     for _ in range(args.warmup):
@@ -345,9 +442,9 @@ def main():
     quads, meshes = batch.totals()
     out = batch.download()
     bad = 0
-    for j in range(n):
-        d = meshdigest.digest(hcm.job_meshes(j, *out))
-        bad += d != golden["cave_region_digest"][j].tobytes()
+    for k in range(n):
+        d = meshdigest.digest(hcm.job_meshes(k, *out))
+        bad += d != golden["cave_region_digest"][perm[k]].tobytes()
     bit_exact = bad == 0 and (out[0]["status"] == 0).all()
 
     plain = None
@@ -382,7 +479,7 @@ def main():
         "config": {"workload": "cave-terrain batch: 4096 chunks of 16^3 (16x16x16-chunk region, 256^3 voxels "
                                "+ rim), seed 2, one persistent launch per step",
                    "chunks": n, "quads_per_step": int(quads), "meshes_per_step": int(meshes),
-                   "threads": tuning["threads"], "wg_per_cu": tuning["wg_per_cu"]},
+                   "threads": tuning["threads"], "wg_per_cu": tuning["wg_per_cu"], "job_order": tuning["order"]},
         "quads_per_s": round(quads * args.steps / dt, 1),
         "bit_exact": bool(bit_exact),
         "plain_per_chunk_chunks_per_s": round(plain, 1) if plain else None,
@@ -392,7 +489,7 @@ def main():
                      "traffic_source": "profiles/mesh_kernel_cave_region.md (FETCH_SIZE + WRITE_SIZE, copy-scaled)",
                      "kernel_ms": round(k_ms, 4), "algorithmic_bytes": int(alg_bytes)},
     }
-    res["draw_lists"] = draw_leg(hcm, workloads, batch, int(meshes))
+    res["draw_lists"] = draw_leg(hcm, workloads, batch, int(meshes), positions[perm])
     batch.close()
     store.close()
     if not args.no_mixed:
@@ -405,13 +502,18 @@ def main():
         res["serving"] = serve_leg(ctx, hcm, workloads, trace)
         log("serving trace with light rebuild")
         res["serving_relight"] = serve_leg(ctx, hcm, workloads, trace, relight=True)
+        log("serving trace at ten times the load rate (200k chunks/s)")
+        res["serving_saturated"] = serve_leg(ctx, hcm, workloads, workloads.spiral_trace(load_rate=200000.0))
     if not args.no_relight:
         log("light rebuild + meshing")
-        res["relight"] = relight_leg(ctx, hcm, workloads, meshdigest, 20, min(16, os.cpu_count() or 1),
+        res["relight"] = relight_leg(ctx, hcm, workloads, meshdigest, 20, min(host_cpus()[0], CPU_SHARE),
                                      not args.no_cpu)
+    if not args.no_dropin:
+        log("drop-in call latency")
+        res["dropin"] = dropin_leg(ctx, hcm, blocks, light, nbr, golden["cave_region_digest"])
     if not args.no_cpu:
         log("CPU baseline")
-        res["cpu_baseline"] = cpu_baseline(blocks, light, nbr, min(16, os.cpu_count() or 1))
+        res["cpu_baseline"] = cpu_baseline(blocks, light, nbr)
     print(json.dumps(res), flush=True)
 
 

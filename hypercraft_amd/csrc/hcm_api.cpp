@@ -486,9 +486,9 @@ static MeshParams base_params(hcm_ctx *c) {
 // Occupancy-derived persistent grid: workgroups per CU times CUs.
 static int persistent_grid(hcm_ctx *c, uint32_t threads) {
 	uint32_t per_cu = c->tuning.wg_per_cu;
-	if (!per_cu) {
-		int lds = mesh_lds_bytes((int)threads);
-		per_cu = lds > 0 ? std::max(1, (160 * 1024) / lds) : 1;
+	if (!per_cu) { // as many as the CU's LDS holds: 128 blocks of 1,280 bytes
+		const int lds = mesh_lds_bytes((int)threads);
+		per_cu = lds > 0 ? std::max(1, 128 / ((lds + 1279) / 1280)) : 1;
 		per_cu = std::min<uint32_t>(per_cu, 2048 / threads);
 	}
 	return (int)(per_cu * (uint32_t)c->n_cu);
@@ -552,7 +552,7 @@ static int mesh_halo_locked(hcm_ctx *ctx, hcm_host_result *out) {
 	hipStream_t st = ctx->stream;
 	HCM_HIP(hipMemcpyAsync(ctx->d_halo_in, ctx->h_halo_in, kHaloBytes, hipMemcpyHostToDevice, st));
 	HCM_HIP(launch_halo_unpack(reinterpret_cast<const uint16_t *>(ctx->d_halo_in), ctx->d_halo_in + HCM_HALO_BLOCKS * 2,
-	                           ctx->d_pblocks, ctx->d_plight, st));
+	                           1, ctx->d_pblocks, ctx->d_plight, st));
 	HCM_HIP(launch_reset(ctx->d_pctr, st));
 	MeshParams p = base_params(ctx);
 	p.blocks = ctx->d_pblocks;

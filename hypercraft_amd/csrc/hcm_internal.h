@@ -126,8 +126,8 @@ struct VoxelEdit {
 // Launchers (hcm_kernels.hip)
 hipError_t launch_reset(Counters *ctr, hipStream_t stream);
 hipError_t launch_mesh(const MeshParams &p, int threads, int grid, hipStream_t stream);
-hipError_t launch_halo_unpack(const uint16_t *hblocks, const uint8_t *hlight, uint16_t *blocks, uint8_t *light,
-                              hipStream_t stream);
+hipError_t launch_halo_unpack(const uint16_t *hblocks, const uint8_t *hlight, uint32_t n, uint16_t *blocks,
+                              uint8_t *light, hipStream_t stream);
 hipError_t launch_edits(uint16_t *blocks, uint8_t *light, int8_t *heights, const VoxelEdit *edits, uint32_t n,
                         hipStream_t stream);
 int mesh_lds_bytes(int threads);

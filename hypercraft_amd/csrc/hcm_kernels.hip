@@ -36,7 +36,9 @@ constexpr int kHalo = 18;
 constexpr int kHaloVox = kHalo * kHalo * kHalo;
 constexpr int kHaloRows = kHalo * kHalo;
 constexpr int kTasks = 102; // 3 axes x 17 slices x 2 directions
-constexpr int kListCap = 1536;
+// Entries of the face-word stage. 1,472 keeps the 256-thread workgroup's LDS within three per CU
+// (static_assert below); the tasks of a chunk with more visible faces are taken in several groups.
+constexpr int kListCap = 1472;
 constexpr uint32_t kSplitQuads = 16383u; // a BlockMesh holds at most 65,532 vertices (:273-279, :397-403)
 constexpr int kMaxMeshPerPass = 8;
 
@@ -113,15 +115,14 @@ template <int NT> struct Smem {
 	alignas(16) uint16_t ev[kTasks][16];
 	alignas(16) uint16_t tp[kTasks][16];
 	uint2 key[kListCap];       // face words of the current group of tasks, in dense order
-	uint32_t fpre[kTasks + 2]; // prefix of visible faces per task (the dense face order)
+	uint16_t fpre[kTasks + 2]; // prefix of visible faces per task (the dense face order; at most 24,576)
 	uint32_t qpre_o[kTasks];   // per task: index of its first opaque quad in the opaque pass
 	uint32_t qpre_t[kTasks];   // per task: index of its first transparent quad in the transparent pass
-	uint32_t epre[kTasks];     // per task: position of its first quad in the emission order (mesh splits)
 	uint2 rows[kMaxPid];
 	uint32_t pid_vox[kMaxPid];
 	uint32_t codes[kMaxCodes];
 	uint32_t custom_bits[128]; // in-chunk cells with a custom mesh
-	uint8_t pid_id[256];       // texture row by block id when the meta byte does not change it, else 0xff
+	uint32_t tmpl_id[256];     // voxel-word template by block id when the meta byte does not change it, else kTmplMeta
 	uint32_t aabb[2][kMaxMeshPerPass][6];
 	uint32_t events[16][2];
 	uint64_t scan[NT / 64 + 1];
@@ -152,9 +153,13 @@ template <int NT> __device__ __forceinline__ uint32_t row_code(const Smem<NT> &s
 }
 
 // Texture row of a block value: from LDS for most ids, from the full table for meta-dependent ones.
+// A template word flag (bit 15 of the voxel word is otherwise unused): the block id's texture row depends
+// on the meta byte, so the full table decides.
+constexpr uint32_t kTmplMeta = 1u << 15;
+
 template <int NT> __device__ __forceinline__ uint32_t block_pid(const MeshParams &p, const Smem<NT> &s, uint32_t b) {
-	const uint32_t q = s.pid_id[b & 0xffu];
-	return q != 0xffu ? q : p.reg.pid_of[b];
+	const uint32_t q = s.tmpl_id[b & 0xffu];
+	return q & kTmplMeta ? p.reg.pid_of[b] : vox_pid(q);
 }
 
 __device__ __forceinline__ uint32_t face_class(uint32_t vox, int face) {
@@ -926,7 +931,8 @@ __device__ __forceinline__ void quad_done(Smem<NT> &s, uint32_t pass, uint32_t m
 	}
 }
 
-// The quad that starts at the visible face with dense index gi and face word `word`, if one does. Per
+// The quad that starts at the visible face at desc (task t | row u << 7 | column v << 11) with face word
+// `word`, if one does. Per
 // task, ne still holds the visible cells and the counting pass left the opaque quad starts in eh, the
 // transparent ones in tp and the uncovered visible cells S in ev; the covered cells are C = ne & ~S. A
 // quad from (u, v0) spans S's run from v0 to the next start along V, and covers rows u .. u + h - 1 of its
@@ -934,10 +940,9 @@ __device__ __forceinline__ void quad_done(Smem<NT> &s, uint32_t pass, uint32_t m
 // is either not covered or starts a quad. Its index in its pass follows the pass's starts in the task's
 // earlier rows and columns (the reference's order: rows, then columns).
 template <int NT>
-__device__ __forceinline__ void emit_face(const MeshParams &p, Smem<NT> &s, int t0, int t1, uint32_t gi, uint2 word,
-                                          uint32_t n0, uint32_t pool_base, uint32_t box[2][6]) {
-	int t, u, v;
-	face_pos(s, t0, t1, gi, t, u, v);
+__device__ __forceinline__ void emit_face(const MeshParams &p, Smem<NT> &s, uint32_t desc, uint2 word, uint32_t n0,
+                                          uint32_t pool_base, uint32_t box[2][6]) {
+	const int t = desc & 127, u = (desc >> 7) & 15, v = (desc >> 11) & 15;
 	const uint32_t so = s.eh[t][u], st = s.tp[t][u], starts = so | st;
 	if (!((starts >> v) & 1u))
 		return;
@@ -965,7 +970,7 @@ __device__ __forceinline__ void emit_face(const MeshParams &p, Smem<NT> &s, int 
 	if (mj == 0 && mk > 0) { // emission position, only needed for a split: both passes' starts before it
 		const uint64_t *o64 = reinterpret_cast<const uint64_t *>(s.eh[t]);
 		const uint64_t *t64 = reinterpret_cast<const uint64_t *>(s.tp[t]);
-		epos = s.epre[t] + __popc(starts & below);
+		epos = s.qpre_o[t] + s.qpre_t[t] + __popc(starts & below); // custom quads, then the earlier tasks'
 		for (int q = 0; q < 4; ++q) {
 			const int nb = 16 * (u - 4 * q);
 			const uint64_t m = nb >= 64 ? ~0ull : (nb > 0 ? (1ull << nb) - 1ull : 0ull);
@@ -996,9 +1001,10 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 	constexpr int CPT = HCM_CHUNK_VOXELS / NT; // custom cells per thread
 
-	// 1. halo staging: one thread per x-row (hy, hz) of 18 voxels; the lookups of a row are issued as
-	// batches (block id -> texture row in LDS, the rare meta-dependent ids from the full table, then the
-	// rows' voxel-word templates), not as one dependent chain per voxel
+	// 1. halo staging: one thread per x-row (hy, hz) of 18 voxels. A voxel's word is its light fields and
+	// the template its block id's byte looks up in LDS (the full table only for the few ids whose meta byte
+	// changes the texture row); the row's flags come from an AND and an OR over its templates and from
+	// comparing packed block pairs, the custom-mesh cells only when some template has a mesh.
 	uint32_t my_flags = 0, wave_ref = 0xffffffffu;
 	for (int row = tid; row < kHaloRows; row += NT) {
 		const int hz = row % kHalo, hy = row / kHalo, y = hy - 1, z = hz - 1;
@@ -1009,75 +1015,89 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 		const uint4 b0 = *reinterpret_cast<const uint4 *>(p.blocks + cm);
 		const uint4 b1 = *reinterpret_cast<const uint4 *>(p.blocks + cm + 8);
 		const uint32_t bw[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
-		uint32_t blk[18], lt[18];
-		blk[0] = p.blocks[cl], blk[17] = p.blocks[cr];
-#pragma unroll
-		for (int x = 0; x < 16; ++x)
-			blk[x + 1] = (bw[x >> 1] >> (16 * (x & 1))) & 0xffffu;
+		const uint32_t bl = p.blocks[cl], br = p.blocks[cr];
+		uint32_t lw[5];
 		if (p.halo_light) { // the job's rebuilt light, one 20-byte record per halo x-row
 			const uint32_t *hl = reinterpret_cast<const uint32_t *>(p.halo_light + (size_t)job * kLightJobBytes +
 			                                                        row * kLightRowBytes);
-			const uint32_t lw[5] = {hl[0], hl[1], hl[2], hl[3], hl[4]};
 #pragma unroll
-			for (int x = 0; x < 18; ++x)
-				lt[x] = (lw[x >> 2] >> (8 * (x & 3))) & 0xffu;
-		} else {
+			for (int k = 0; k < 5; ++k)
+				lw[k] = hl[k];
+		} else { // the same layout from the store: x = -1 in byte 0, x = 0 .. 15 in bytes 1 .. 16, x = 16 in 17
 			const uint4 lm = *reinterpret_cast<const uint4 *>(p.light + cm);
-			const uint32_t lw[4] = {lm.x, lm.y, lm.z, lm.w};
-			lt[0] = p.light[cl], lt[17] = p.light[cr];
-#pragma unroll
-			for (int x = 0; x < 16; ++x)
-				lt[x + 1] = (lw[x >> 2] >> (8 * (x & 3))) & 0xffu;
+			const uint32_t ll = p.light[cl], lr = p.light[cr];
+			lw[0] = ll | (lm.x << 8);
+			lw[1] = (lm.x >> 24) | (lm.y << 8);
+			lw[2] = (lm.y >> 24) | (lm.z << 8);
+			lw[3] = (lm.z >> 24) | (lm.w << 8);
+			lw[4] = (lm.w >> 24) | (lr << 8);
 		}
 		// one reference value per wavefront (its first row's first voxel): a halo of one block value has
 		// every voxel equal to it and all wavefronts' references equal
-		const uint32_t v0 = __builtin_amdgcn_readfirstlane(blk[0]);
+		const uint32_t v0 = __builtin_amdgcn_readfirstlane(bl);
 		if (row < NT)
 			wave_ref = v0;
-		uint32_t pids[18];
-		bool meta_dependent = false;
+		const uint32_t ref2 = wave_ref | (wave_ref << 16);
+		bool nonuniform = bl != wave_ref || br != wave_ref;
 #pragma unroll
-		for (int x = 0; x < 18; ++x) {
-			pids[x] = s.pid_id[blk[x] & 0xffu];
-			meta_dependent |= pids[x] == 0xffu;
-		}
-		if (meta_dependent) {
+		for (int k = 0; k < 8; ++k)
+			nonuniform |= bw[k] != ref2;
+		uint32_t tm[18];
+		tm[0] = s.tmpl_id[bl & 0xffu];
+		tm[17] = s.tmpl_id[br & 0xffu];
 #pragma unroll
-			for (int x = 0; x < 18; ++x)
-				if (pids[x] == 0xffu)
-					pids[x] = p.reg.pid_of[blk[x]];
-		}
-		uint32_t pv[18];
+		for (int x = 0; x < 16; ++x)
+			tm[x + 1] = s.tmpl_id[(bw[x >> 1] >> (16 * (x & 1))) & 0xffu];
+		uint32_t any_meta = 0;
 #pragma unroll
 		for (int x = 0; x < 18; ++x)
-			pv[x] = s.pid_vox[pids[x]];
-		const bool inside_yz = (unsigned)y < 16u && (unsigned)z < 16u;
-		uint32_t mesh_bits = 0, mixed = 0, inner = 0;
+			any_meta |= tm[x];
+		if (any_meta & kTmplMeta) { // ids whose texture row depends on the meta byte
+			// every voxel's row from the full table (any 16-bit value indexes it), issued as one batch of
+			// loads and then one of LDS lookups, so the row waits once on memory and not once per voxel
+			uint32_t full[18];
+#pragma unroll
+			for (int x = 0; x < 18; ++x) {
+				const uint32_t b = x == 0 ? bl : (x == 17 ? br : (bw[(x - 1) >> 1] >> (16 * ((x - 1) & 1))) & 0xffffu);
+				full[x] = p.reg.pid_of[b];
+			}
+#pragma unroll
+			for (int x = 0; x < 18; ++x) {
+				const uint32_t v = s.pid_vox[full[x]];
+				tm[x] = tm[x] & kTmplMeta ? v : tm[x];
+			}
+		}
+		uint32_t and_all = ~0u, or_inner = 0;
 		uint32_t *dst = s.vox + row * kHalo;
 #pragma unroll
 		for (int x = 0; x < 18; ++x) {
-			dst[x] = vox_light(lt[x]) | pv[x];
-			mixed |= (pv[x] ^ kVoxOpaqueCube) & kVoxShape;
+			dst[x] = vox_light((lw[x >> 2] >> (8 * (x & 3))) & 0xffu) | tm[x];
+			and_all &= tm[x];
 			if (x >= 1 && x <= 16)
-				inner |= pv[x] & kVoxShape;
-			if (blk[x] != wave_ref)
-				my_flags |= kFlagNonUniform;
-			if (pv[x] & kVoxMesh) {
-				if (inside_yz && x >= 1 && x <= 16)
-					mesh_bits |= 1u << (x - 1);
-				if (pv[x] & kVoxFlow)
-					my_flags |= kFlagFlow;
-			}
+				or_inner |= tm[x];
 		}
-		if (mesh_bits) {
+		const uint32_t or_all = or_inner | tm[0] | tm[17];
+		const bool inside_yz = (unsigned)y < 16u && (unsigned)z < 16u;
+		// every voxel an opaque cube: the shape bits that the cube has set are set in all, the others in none
+		const bool mixed = (and_all & kVoxShape & kVoxOpaqueCube) != (kVoxShape & kVoxOpaqueCube) ||
+		                   (or_all & kVoxShape & ~kVoxOpaqueCube) != 0;
+		if (nonuniform)
+			my_flags |= kFlagNonUniform;
+		if (mixed)
+			my_flags |= kFlagMixed;
+		if (inside_yz && (or_inner & kVoxShape))
+			my_flags |= kFlagInnerFaces;
+		if (or_all & kVoxFlow)
+			my_flags |= kFlagFlow;
+		if (inside_yz && (or_inner & kVoxMesh)) {
+			uint32_t mesh_bits = 0;
+#pragma unroll
+			for (int x = 0; x < 16; ++x)
+				mesh_bits |= ((tm[x + 1] / kVoxMesh) & 1u) << x;
 			const uint32_t cell0 = 16u * (uint32_t(z) + 16u * uint32_t(y));
 			atomicOr(&s.custom_bits[cell0 >> 5], mesh_bits << (cell0 & 31));
 			my_flags |= kFlagCustom;
 		}
-		if (mixed)
-			my_flags |= kFlagMixed;
-		if (inside_yz && inner)
-			my_flags |= kFlagInnerFaces;
 	}
 	if (my_flags)
 		atomicOr(&s.flags, my_flags);
@@ -1161,6 +1181,7 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 	// face words per group of tasks, then the group's link masks; the last group's words stay in key
 	constexpr int kMine = (kListCap + NT - 1) / NT;
 	int t0_last = 0;
+	uint32_t keep[kMine]; // the positions of this lane's faces of the last group, for the emission
 #pragma unroll 1
 	for (int t0 = 0; t0 < kTasks;) {
 		const int t1 = group_end(s, t0);
@@ -1185,6 +1206,9 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 				mine[j] = mine[j + 1];
 			mine[kMine - 1] = d;
 		}
+#pragma unroll
+		for (int j = 0; j < kMine; ++j)
+			keep[j] = mine[j];
 		__syncthreads();
 #pragma unroll 2
 		for (int k = 0; k < kMine; ++k) {
@@ -1213,8 +1237,53 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 			c_o += c.count - nt;
 		}
 	}
-	if (wave * 64 < kTasks)
-		greedy_count(s, tid, tid < kTasks, q_o, q_t);
+	// a task with no equal neighbouring faces (no link bits) makes every visible face a 1 x 1 quad: its
+	// results follow from its masks directly. The others are listed and merged by one lane each, in as few
+	// wavefronts as they fill. The list passes through qpre_t and the counts through qpre_o (o | t << 16)
+	// before either holds its own values.
+	if (tid == 0)
+		s.group_end = 0;
+	__syncthreads();
+	if (tid < kTasks) {
+		const int t = tid;
+		uint64_t any = 0;
+#pragma unroll
+		for (int q = 0; q < 4; ++q)
+			any |= reinterpret_cast<const uint64_t *>(s.eh[t])[q] | reinterpret_cast<const uint64_t *>(s.ev[t])[q];
+		if (any) {
+			s.qpre_t[atomicAdd(&s.group_end, 1u)] = uint32_t(t);
+		} else {
+			uint32_t no = 0, nt = 0;
+#pragma unroll
+			for (int q = 0; q < 4; ++q) {
+				uint64_t *tp64 = reinterpret_cast<uint64_t *>(s.tp[t]) + q;
+				const uint64_t ne = reinterpret_cast<const uint64_t *>(s.ne[t])[q], tp = *tp64 & ne;
+				reinterpret_cast<uint64_t *>(s.ev[t])[q] = ne;       // uncovered visible cells
+				reinterpret_cast<uint64_t *>(s.eh[t])[q] = ne & ~tp; // opaque quads start here
+				*tp64 = tp;                                          // transparent quads start here
+				no += __popcll(ne & ~tp);
+				nt += __popcll(tp);
+			}
+			s.qpre_o[t] = no | nt << 16;
+		}
+	}
+	__syncthreads();
+	{
+		const uint32_t n_link = s.group_end;
+		if (wave * 64 < (int)n_link) {
+			const bool active = tid < (int)n_link;
+			const int t = active ? (int)s.qpre_t[tid] : 0;
+			uint32_t no, nt;
+			greedy_count(s, t, active, no, nt);
+			if (active)
+				s.qpre_o[t] = no | nt << 16;
+		}
+	}
+	__syncthreads();
+	if (tid < kTasks) {
+		const uint32_t c = s.qpre_o[tid];
+		q_o = c & 0xffffu, q_t = c >> 16;
+	}
 	// 5. block-wide prefix sums and allocation: the chunk's output is [pool_base, +n0) opaque and
 	// [+n0, +n0+n1) transparent; custom quads come first in each pass, then the tasks' quads in task order
 	uint64_t cust_total, task_total;
@@ -1231,7 +1300,6 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 	if (tid < kTasks) {
 		s.qpre_o[tid] = cust0 + (uint32_t)task_pre;
 		s.qpre_t[tid] = cust1 + (uint32_t)(task_pre >> 32);
-		s.epre[tid] = cust0 + cust1 + (uint32_t)task_pre + (uint32_t)(task_pre >> 32);
 	}
 	if (tid == 0) {
 		const uint32_t nm0 = (n0 + kSplitQuads - 1) / kSplitQuads, nm1 = (n1 + kSplitQuads - 1) / kSplitQuads;
@@ -1282,24 +1350,42 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 	// whose face words are still in key, then the earlier groups with their words computed again
 	{
 		const uint32_t g0 = s.fpre[t0_last], n = s.fpre[kTasks] - g0;
-#pragma unroll 1
-		for (uint32_t i = tid; i < n; i += NT)
-			emit_face(p, s, t0_last, kTasks, g0 + i, s.key[i], n0, pool_base, box);
+#pragma unroll 2
+		for (int k = 0; k < kMine; ++k) {
+			const uint32_t i = tid + k * NT;
+			if (i < n)
+				emit_face(p, s, keep[0], s.key[i], n0, pool_base, box);
+#pragma unroll
+			for (int j = 0; j + 1 < kMine; ++j)
+				keep[j] = keep[j + 1];
+		}
 	}
 #pragma unroll 1
 	for (int t0 = 0; t0 < t0_last;) {
 		__syncthreads(); // key is read by the previous group's lanes
 		const int t1 = group_end(s, t0);
 		const uint32_t g0 = s.fpre[t0], n = s.fpre[t1] - g0;
-#pragma unroll 1
-		for (uint32_t i = tid; i < n; i += NT) {
-			uint32_t d;
-			s.key[i] = face_word(s, t0, t1, g0 + i, d);
+#pragma unroll 2
+		for (int k = 0; k < kMine; ++k) {
+			const uint32_t i = tid + k * NT;
+			uint32_t d = 0;
+			if (i < n)
+				s.key[i] = face_word(s, t0, t1, g0 + i, d);
+#pragma unroll
+			for (int j = 0; j + 1 < kMine; ++j)
+				keep[j] = keep[j + 1];
+			keep[kMine - 1] = d;
 		}
 		__syncthreads();
-#pragma unroll 1
-		for (uint32_t i = tid; i < n; i += NT)
-			emit_face(p, s, t0, t1, g0 + i, s.key[i], n0, pool_base, box);
+#pragma unroll 2
+		for (int k = 0; k < kMine; ++k) {
+			const uint32_t i = tid + k * NT;
+			if (i < n)
+				emit_face(p, s, keep[0], s.key[i], n0, pool_base, box);
+#pragma unroll
+			for (int j = 0; j + 1 < kMine; ++j)
+				keep[j] = keep[j + 1];
+		}
 		t0 = t1;
 	}
 
@@ -1358,7 +1444,10 @@ template <int NT> __global__ __launch_bounds__(NT, NT >= 512 ? 2 : 3) void mesh_
 	for (int i = threadIdx.x; i < kMaxCodes; i += NT)
 		s.codes[i] = i < (int)p.reg.n_codes ? p.reg.codes[i] : (kCodeEmpty | kCodeTransparent) << 16;
 	for (int i = threadIdx.x; i < 256; i += NT)
-		s.pid_id[i] = p.reg.pid_of[65536 + i];
+	{
+		const uint32_t q = p.reg.pid_of[65536 + i];
+		s.tmpl_id[i] = q == 0xffu ? kTmplMeta : p.reg.pid_vox[q];
+	}
 	if (!p.persistent) {
 		const uint32_t job = p.job_base + blockIdx.x;
 		if (job < p.n_jobs && threadIdx.x < 27)
@@ -1454,25 +1543,30 @@ hipError_t launch_edits(uint16_t *blocks, uint8_t *light, int8_t *heights, const
 	return hipGetLastError();
 }
 
-// Scatters a packed chunk halo (include/hcm.h, HCM_HALO_BLOCKS / HCM_HALO_LIGHT) into a 27-slot store in
-// neighbour-slot order, where the mesher reads it; cells of the slots outside the halo are not written
-// (the mesher never reads them).
+// Scatters n packed chunk halos (include/hcm.h, HCM_HALO_BLOCKS / HCM_HALO_LIGHT; halo j at j times those
+// sizes) into 27 store slots each, job j's chunks at slots 27 j .. 27 j + 26 in neighbour-slot order, where
+// the mesher reads them; cells of the slots outside the halo are not written (the mesher never reads them).
+constexpr int kUnpackBlocksPerHalo = (HCM_HALO_BLOCKS + 255) / 256;
 __global__ void halo_unpack_kernel(const uint16_t *hblocks, const uint8_t *hlight, uint16_t *blocks, uint8_t *light) {
-	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	const int job = blockIdx.x / kUnpackBlocksPerHalo;
+	const int i = (blockIdx.x % kUnpackBlocksPerHalo) * 256 + threadIdx.x;
+	const size_t base = (size_t)job * 27;
 	if (i < HCM_HALO_BLOCKS) {
 		const int x = i % 20 - 2, z = (i / 20) % 20 - 2, y = i / 400 - 1;
-		blocks[(size_t)slot27(x, y, z) * HCM_CHUNK_VOXELS + local16(x, y, z)] = hblocks[i];
+		blocks[(base + slot27(x, y, z)) * HCM_CHUNK_VOXELS + local16(x, y, z)] = hblocks[(size_t)job * HCM_HALO_BLOCKS + i];
 	}
 	if (i < HCM_HALO_LIGHT) {
 		const int x = i % 18 - 1, z = (i / 18) % 18 - 1, y = i / 324 - 1;
-		light[(size_t)slot27(x, y, z) * HCM_CHUNK_VOXELS + local16(x, y, z)] = hlight[i];
+		light[(base + slot27(x, y, z)) * HCM_CHUNK_VOXELS + local16(x, y, z)] = hlight[(size_t)job * HCM_HALO_LIGHT + i];
 	}
 }
 
-hipError_t launch_halo_unpack(const uint16_t *hblocks, const uint8_t *hlight, uint16_t *blocks, uint8_t *light,
-                              hipStream_t stream) {
-	hipLaunchKernelGGL(halo_unpack_kernel, dim3((HCM_HALO_BLOCKS + 255) / 256), dim3(256), 0, stream, hblocks, hlight,
-	                   blocks, light);
+hipError_t launch_halo_unpack(const uint16_t *hblocks, const uint8_t *hlight, uint32_t n, uint16_t *blocks,
+                              uint8_t *light, hipStream_t stream) {
+	if (n == 0)
+		return hipSuccess;
+	hipLaunchKernelGGL(halo_unpack_kernel, dim3(n * kUnpackBlocksPerHalo), dim3(256), 0, stream, hblocks, hlight, blocks,
+	                   light);
 	return hipGetLastError();
 }
 
@@ -1497,6 +1591,10 @@ hipError_t launch_mesh(const MeshParams &p, int threads, int grid, hipStream_t s
 	}
 	return hipGetLastError();
 }
+
+// gfx950 allocates a CU's 160 KB of LDS in 1,280-byte blocks: three 256-thread workgroups per CU need
+// at most 42 blocks each (a 54,208-byte build ran at two per CU).
+static_assert(sizeof(Smem<256>) <= 42 * 1280, "three 256-thread workgroups per CU");
 
 int mesh_lds_bytes(int threads) {
 	switch (threads) {

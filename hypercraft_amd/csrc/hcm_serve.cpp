@@ -70,6 +70,8 @@ constexpr uint64_t kMinArena = 1 << 16; // quads; more than any one chunk's mesh
 
 struct Chunk {
 	uint32_t slot = 0;
+	uint32_t present = 0;           // chunks of its 3x3x3 neighbourhood in the world, itself included
+	uint32_t nbr[27] = {};          // their slots in neighbour-slot order, once all 27 are present
 	bool stale = false;             // mesh out of date and not in a launched batch
 	bool urgent = false;            // stale because of an edit: queued with high priority
 	bool has_mesh = false;
@@ -94,6 +96,7 @@ struct Request {
 	uint32_t slot = 0;             // load: the chunk's device slot
 	uint64_t epoch = 0;            // set when the scheduler applies it
 	std::vector<uint64_t> targets; // chunks it made stale
+	size_t targets_done = 0;       // leading targets already meshed at or after its epoch
 };
 
 struct Upload {
@@ -174,7 +177,8 @@ struct hcm_server {
 	std::deque<uint64_t> stale_fifo; // stale chunks by time (loads, sunlight changes)
 	std::deque<uint64_t> stale_hi;   // stale chunks an edit remeshes: taken first
 	bool pool_blocked = false;       // an arena did not fit while batches ran: wait for one to finish
-	std::vector<std::unique_ptr<Request>> waiting;
+	std::deque<std::unique_ptr<Request>> waiting; // applied requests not yet complete, in epoch order
+	bool completions_due = false;                 // something happened that may complete a waiting request
 	std::deque<Upload> uploads;
 	std::map<uint64_t, PendingEdit> edits; // (slot << 13 | local) -> last edit of that voxel or height
 	uint64_t epoch = 0, batches = 0, applied_done = 0;
@@ -201,14 +205,35 @@ struct hcm_server {
 		out_cv.notify_all();
 	}
 
-	bool meshable(const int32_t p[3]) const {
+	// A new chunk counts its present neighbours and is counted by them; a chunk whose count reaches 27
+	// takes its neighbours' slots (fixed from their first load) once, for every batch that meshes it.
+	void add_chunk(uint64_t key, Chunk &c) {
+		int32_t p[3];
+		key_pos(key, p);
+		auto complete = [&](uint64_t k, Chunk &n) {
+			if (++n.present < 27)
+				return;
+			int32_t q[3];
+			key_pos(k, q);
+			for (int s = 0; s < 27; ++s) {
+				int d[3];
+				slot_offset(s, d);
+				n.nbr[s] = world.at(chunk_key(q[0] + d[0], q[1] + d[1], q[2] + d[2])).slot;
+			}
+		};
 		for (int s = 0; s < 27; ++s) {
 			int d[3];
 			slot_offset(s, d);
-			if (!world.count(chunk_key(p[0] + d[0], p[1] + d[1], p[2] + d[2])))
-				return false;
+			if (d[0] == 0 && d[1] == 0 && d[2] == 0)
+				continue;
+			const uint64_t nk = chunk_key(p[0] + d[0], p[1] + d[1], p[2] + d[2]);
+			auto it = world.find(nk);
+			if (it != world.end()) {
+				complete(nk, it->second);
+				++c.present;
+			}
 		}
-		return true;
+		complete(key, c);
 	}
 	// A chunk made stale is queued once per priority: a chunk already queued by time moves ahead when an
 	// edit marks it (ChunkMeshTask.inl:40 keeps the high priority flag until the task runs); the `stale`
@@ -224,10 +249,9 @@ struct hcm_server {
 		}
 	}
 	void mark_stale(int32_t cx, int32_t cy, int32_t cz, std::vector<uint64_t> &targets, bool urgent) {
-		const int32_t p[3] = {cx, cy, cz};
 		const uint64_t key = chunk_key(cx, cy, cz);
 		auto it = world.find(key);
-		if (it == world.end() || !meshable(p))
+		if (it == world.end() || it->second.present < 27)
 			return;
 		queue_stale(key, it->second, urgent);
 		targets.push_back(key);
@@ -244,6 +268,7 @@ struct hcm_server {
 				Chunk c;
 				c.slot = slot;
 				it = world.emplace(key, c).first;
+				add_chunk(key, it->second);
 			}
 			if (relight()) {
 				it->second.blocks = r->blocks;
@@ -269,6 +294,7 @@ struct hcm_server {
 				// byte of an edit is not used when light is rebuilt. (A light-store server keeps no host copy
 				// of the voxels and applies every edit.)
 				waiting.push_back(std::move(r));
+				completions_due = true;
 				return;
 			}
 			auto pend = [&](const VoxelEdit &ve) {
@@ -330,6 +356,7 @@ struct hcm_server {
 			r->targets.erase(std::unique(r->targets.begin(), r->targets.end()), r->targets.end());
 		}
 		waiting.push_back(std::move(r));
+		completions_due = true;
 	}
 
 	bool any_busy() const {
@@ -449,15 +476,8 @@ struct hcm_server {
 		}
 
 		if (!b.jobs.empty()) {
-			for (size_t j = 0; j < b.jobs.size(); ++j) {
-				int32_t p[3];
-				key_pos(b.jobs[j], p);
-				for (int s = 0; s < 27; ++s) {
-					int d[3];
-					slot_offset(s, d);
-					b.h_nbr[27 * j + s] = world.at(chunk_key(p[0] + d[0], p[1] + d[1], p[2] + d[2])).slot;
-				}
-			}
+			for (size_t j = 0; j < b.jobs.size(); ++j)
+				std::memcpy(b.h_nbr + 27 * j, world.at(b.jobs[j]).nbr, 27 * sizeof(uint32_t));
 			HCM_CHECK(hipMemcpyAsync(b.d_nbr, b.h_nbr, b.jobs.size() * 27 * 4, hipMemcpyHostToDevice, s_comp));
 			HCM_CHECK(hipEventRecord(b.ev_k0, s_comp));
 			if (relight()) { // rebuild the jobs' light from the slots they read (ChunkMeshTask.cpp:48-111)
@@ -600,28 +620,33 @@ struct hcm_server {
 			stats.gpu_ms += ms;
 		}
 		applied_done = std::max(applied_done, b.epoch);
+		completions_due = true;
 		b.busy = false;
 		pool_blocked = false; // its arena's unused part and replaced meshes went back to the pool
 		return HCM_OK;
 	}
 
+	// Runs after a batch finished or requests were applied. Only requests up to applied_done can complete:
+	// they lead the epoch-ordered waiting list. A chunk's meshed_epoch only grows (batches retire in launch
+	// order), so a request resumes its target check where the last pass stopped.
 	void complete_ready() {
+		if (!completions_due)
+			return;
+		completions_due = false;
 		// with no data change pending or in flight, the device holds every request's data (a request that
 		// changed nothing, such as an edit that sets a voxel to its block, then completes at once)
 		if (uploads.empty() && edits.empty() && !any_busy())
 			applied_done = epoch;
 		const auto now = Clock::now();
 		std::vector<hcm_completion> out;
+		size_t n_eligible = 0;
 		for (auto &r : waiting) {
 			if (r->epoch > applied_done)
-				continue;
-			bool ok = true;
-			for (uint64_t key : r->targets)
-				if (world.at(key).meshed_epoch < r->epoch) {
-					ok = false;
-					break;
-				}
-			if (!ok)
+				break;
+			++n_eligible;
+			while (r->targets_done < r->targets.size() && world.at(r->targets[r->targets_done]).meshed_epoch >= r->epoch)
+				++r->targets_done;
+			if (r->targets_done < r->targets.size())
 				continue;
 			hcm_completion c;
 			c.tag = r->tag;
@@ -632,7 +657,8 @@ struct hcm_server {
 			out.push_back(c);
 			r.reset();
 		}
-		waiting.erase(std::remove(waiting.begin(), waiting.end(), nullptr), waiting.end());
+		if (!out.empty())
+			waiting.erase(std::remove(waiting.begin(), waiting.begin() + n_eligible, nullptr), waiting.begin() + n_eligible);
 		if (!out.empty()) {
 			std::lock_guard<std::mutex> lk(out_mu);
 			for (auto &c : out)

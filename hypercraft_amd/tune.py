@@ -1,10 +1,14 @@
-"""Autotuner for the batched meshing kernel: workgroup size x workgroups per CU.
+"""Autotuner for the batched meshing kernel: workgroup size x workgroups per CU x job order.
 
 The persistent kernel's cost per chunk depends on the workgroup size (how many lanes share the
-102 visibility tasks, the light-word pass and the emit loop) and on how many workgroups each CU
-holds (LDS occupancy vs latency hiding). `autotune` times every candidate on the caller's batch with
-HIP events, keeps only settings whose output is bit-identical to the first candidate's, applies the
-fastest to the context, and can cache the choice in a JSON file keyed by device and workload.
+102 visibility tasks, the face-word pass and the emission), on how many workgroups each CU holds
+(LDS occupancy vs latency hiding), and on the order of the job list: the kernel cuts the list into
+eight contiguous parts, one per XCD, so the order decides which chunks share an XCD's L2 (a chunk's
+halo is read by its 26 neighbours' jobs too). `autotune` times every candidate on the caller's jobs
+with HIP events, keeps only settings whose meshes equal the wanted digests (the reference's, when the
+caller has them; else the first candidate's), applies the fastest to the context and returns it with
+the job order to build batches with. The choice can be cached in a JSON file keyed by device, the
+library build and the workload; a cached choice is checked against the digests before it is used.
 """
 import hashlib
 import json
@@ -14,69 +18,125 @@ import numpy as np
 
 THREADS = (128, 256, 512)
 WG_PER_CU = (0, 1, 2, 3, 4)  # 0: as many as the LDS footprint allows
+ORDERS = ("linear", "morton")
 
 
-def _output_digest(batch):
-    """Digest of every job's mesh list in job order (pool positions vary from run to run)."""
+def job_order(name, positions):
+    """Permutation of the jobs: "linear" keeps the caller's order; "morton" sorts the jobs along the
+    3-D Z-order curve of their chunk positions, so each eighth of the list (one XCD's part) is a compact
+    block of space and consecutive jobs are neighbours."""
+    p = np.asarray(positions, np.int64)
+    if name == "linear":
+        return np.arange(p.shape[0])
+    p = p - p.min(0)
+    code = np.zeros(p.shape[0], np.int64)
+    for bit in range(21):
+        for a in range(3):
+            code |= ((p[:, a] >> bit) & 1) << (3 * bit + a)
+    return np.argsort(code, kind="stable")
+
+
+def job_digests(batch):
+    """Per job, the digest of its mesh list (tests/meshdigest.py's layout), in the batch's job order."""
+    from . import hcm
+
     res, recs, verts, idx = batch.download()
-    h = hashlib.sha256()
-    for f in ("status", "mesh_count", "quad_count"):
-        h.update(np.ascontiguousarray(res[f]).tobytes())
-    for r in res:
-        for m in recs[r["first_mesh"]:r["first_mesh"] + r["mesh_count"]]:
-            q0, n = int(m["first_quad"]), int(m["quad_count"])
-            h.update(m["aabb"].tobytes())
-            h.update(bytes([int(m["transparent"])]))
-            h.update(verts[4 * q0:4 * (q0 + n)].tobytes())
-            h.update(idx[6 * q0:6 * (q0 + n)].tobytes())
-    return h.hexdigest()
+    out = np.zeros((res.shape[0], 32), np.uint8)
+    for j in range(res.shape[0]):
+        h = hashlib.sha256()
+        for m in hcm.job_meshes(j, res, recs, verts, idx):
+            hdr = np.array([m.vertices.shape[0], m.indices.shape[0], *m.aabb, m.transparent], np.uint32)
+            h.update(hdr.tobytes())
+            h.update(np.ascontiguousarray(m.vertices, np.uint32).tobytes())
+            h.update(np.ascontiguousarray(m.indices, np.uint16).tobytes())
+        out[j] = np.frombuffer(h.digest(), np.uint8)
+    return out
+
+
+def library_id():
+    """A hash of the loaded library build: cached choices are valid for the build that made them."""
+    from . import hcm
+
+    with open(hcm.LIB_PATH, "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()[:16]
 
 
 def _cache_key(device, workload):
-    return f"{device}|{workload}"
+    return f"{device}|{library_id()}|{workload}"
 
 
-def autotune(ctx, batch, workload="", reps=5, cache_path=None, candidates=None, log=None):
-    """Times the candidates on `batch` and applies the fastest bit-identical one to `ctx`.
+def _time(ctx, batch, threads, wg, reps):
+    ctx.set_tuning(wg, threads)
+    batch.run()
+    batch.sync()
+    ms = []
+    for _ in range(reps):
+        batch.run()
+        ms.append(batch.sync())
+    return float(np.median(ms))
 
-    Returns {"threads", "wg_per_cu", "kernel_ms", "table": [(threads, wg_per_cu, ms), ...]}.
-    """
+
+def autotune(ctx, store, nbr, workload="", positions=None, want=None, reps=5, cache_path=None, candidates=None,
+             log=None):
+    """Times the candidates (threads, wg_per_cu, order) on the jobs `nbr` over `store` and applies the
+    fastest whose meshes equal `want` (per-job digests in the caller's job order) to `ctx`.
+
+    Returns {"threads", "wg_per_cu", "order", "perm", "kernel_ms", "table": [(threads, wg, order, ms, ok)]};
+    build the batch as hcm.Batch(ctx, store, nbr[perm]) (job k of it is the caller's job perm[k])."""
+    from . import hcm
+
     device = "gfx950"  # the library runs on no other target
+    orders = {o: job_order(o, positions) for o in (ORDERS if positions is not None else ("linear",))}
     if cache_path and os.path.exists(cache_path):
         with open(cache_path) as f:
             cached = json.load(f).get(_cache_key(device, workload))
-        if cached:
-            ctx.set_tuning(cached["wg_per_cu"], cached["threads"])
-            return dict(cached, cached=True)
-    candidates = candidates or [(t, w) for t in THREADS for w in WG_PER_CU]
+        if cached and cached["order"] in orders:
+            perm = orders[cached["order"]]
+            batch = hcm.Batch(ctx, store, nbr[perm])
+            try:
+                ctx.set_tuning(cached["wg_per_cu"], cached["threads"])
+                batch.run()
+                batch.sync()
+                ok = want is None or np.array_equal(job_digests(batch), want[perm])
+            finally:
+                batch.close()
+            if ok:
+                return dict(cached, perm=perm, cached=True)
+            if log:
+                log("tune: the cached choice no longer gives the wanted meshes; tuning again")
+    candidates = candidates or [(t, w, o) for o in orders for t in THREADS for w in WG_PER_CU]
     table, best, ref = [], None, None
-    for threads, wg in candidates:
-        ctx.set_tuning(wg, threads)
-        batch.run()
-        batch.sync()
-        ms = []
-        for _ in range(reps):
-            batch.run()
-            ms.append(batch.sync())
-        d = _output_digest(batch)
-        if ref is None:
-            ref = d
-        t = float(np.median(ms))
-        table.append((threads, wg, round(t, 4), d == ref))
-        if log:
-            log(f"tune: threads {threads} wg/CU {wg or 'auto'}: {t:.4f} ms" + ("" if d == ref else " (output differs!)"))
-        if d == ref and (best is None or t < best[2]):
-            best = (threads, wg, t)
+    batches = {}
+    try:
+        for threads, wg, order in candidates:
+            if order not in batches:
+                batches[order] = hcm.Batch(ctx, store, nbr[orders[order]])
+            batch = batches[order]
+            t = _time(ctx, batch, threads, wg, reps)
+            d = job_digests(batch)[np.argsort(orders[order])]  # back to the caller's job order
+            if want is None and ref is None:
+                ref = d
+            ok = bool(np.array_equal(d, want if want is not None else ref))
+            table.append((threads, wg, order, round(t, 4), ok))
+            if log:
+                log(f"tune: threads {threads} wg/CU {wg or 'auto'} order {order}: {t:.4f} ms" +
+                    ("" if ok else " (output differs!)"))
+            if ok and (best is None or t < best[3]):
+                best = (threads, wg, order, t)
+    finally:
+        for b in batches.values():
+            b.close()
     if best is None:
-        raise RuntimeError("autotune: no candidate produced the reference output")
+        raise RuntimeError("autotune: no candidate produced the wanted meshes")
     ctx.set_tuning(best[1], best[0])
-    out = {"threads": best[0], "wg_per_cu": best[1], "kernel_ms": round(best[2], 4), "table": table}
+    out = {"threads": best[0], "wg_per_cu": best[1], "order": best[2], "kernel_ms": round(best[3], 4),
+           "table": table, "perm": orders[best[2]]}
     if cache_path:
         data = {}
         if os.path.exists(cache_path):
             with open(cache_path) as f:
                 data = json.load(f)
-        data[_cache_key(device, workload)] = {k: out[k] for k in ("threads", "wg_per_cu", "kernel_ms")}
+        data[_cache_key(device, workload)] = {k: out[k] for k in ("threads", "wg_per_cu", "order", "kernel_ms")}
         with open(cache_path, "w") as f:
             json.dump(data, f, indent=1)
     return out
