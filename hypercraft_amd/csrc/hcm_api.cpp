@@ -201,6 +201,18 @@ constexpr size_t kPoutRecs = 64, kPoutVerts = 64 + kPlainRecs * sizeof(hcm_mesh_
 constexpr size_t kPoutIdx = kPoutVerts + kPlainFastQuads * 4 * sizeof(uint2);
 constexpr size_t kPoutBytes = kPoutIdx + kPlainFastQuads * 6 * 2;
 
+// Scratch of hcm_mesh_halos_noalloc: the device memory its kernels use beyond the caller's inputs and
+// outputs -- work counters, the job table (job j reads slots 27 j .. 27 j + 26) and the 27-slot stores
+// its halos are scattered into -- for at most kNoAllocMaxJobs chunks per call, 2.7 MB. Each GPU holds its
+// own copy from the library's load to the end of the process; calls on one GPU share it.
+constexpr uint32_t kNoAllocMaxJobs = 8;
+constexpr size_t kScratchNbr = (sizeof(Counters) + 255) / 256 * 256;
+constexpr size_t kScratchBlocks = kScratchNbr + (kNoAllocMaxJobs * 27 * 4 + 255) / 256 * 256;
+constexpr size_t kScratchLight = kScratchBlocks + (size_t)kNoAllocMaxJobs * 27 * HCM_CHUNK_VOXELS * 2;
+constexpr size_t kScratchBytes = kScratchLight + (size_t)kNoAllocMaxJobs * 27 * HCM_CHUNK_VOXELS;
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+static_assert(kNoAllocMaxJobs == HCM_NOALLOC_MAX_CHUNKS, "documented limit");
+
 } // namespace
 
 struct hcm_ctx {
@@ -238,6 +250,9 @@ struct hcm_ctx {
 	uint8_t *d_halo_in = nullptr;
 	uint8_t *h_halo_in = nullptr;
 	uint8_t *h_pout = nullptr;
+	// this device's copy of `scratch` (hcm_mesh_halos_noalloc); host code uses this address, never the
+	// array's name, which on the host is only a stand-in
+	uint8_t *d_scratch = nullptr;
 	// plain path light rebuild
 	int8_t *d_pheights = nullptr;
 	int8_t *h_pheights = nullptr;
@@ -430,6 +445,13 @@ static int create_impl(hcm_ctx *c, const hcm_registry *registry, uint64_t pool_q
 	for (uint32_t i = 0; i < 27; ++i)
 		ident[i] = i;
 	HCM_HIP(hipMemcpy(c->d_pnbr, ident, sizeof(ident), hipMemcpyHostToDevice));
+	// the no-allocation entry point's scratch on this (current) device, and its constant job table; every
+	// context on the device writes the same table
+	HCM_HIP(hipGetSymbolAddress(reinterpret_cast<void **>(&c->d_scratch), HIP_SYMBOL(scratch)));
+	uint32_t jobs[kNoAllocMaxJobs * 27];
+	for (uint32_t i = 0; i < kNoAllocMaxJobs * 27; ++i)
+		jobs[i] = i;
+	HCM_HIP(hipMemcpy(c->d_scratch + kScratchNbr, jobs, sizeof(jobs), hipMemcpyHostToDevice));
 	return HCM_OK;
 }
 
@@ -618,6 +640,40 @@ extern "C" int hcm_mesh_halo(hcm_ctx *ctx, const uint16_t *blocks, const uint8_t
 	std::memcpy(ctx->h_halo_in, blocks, HCM_HALO_BLOCKS * 2);
 	std::memcpy(ctx->h_halo_in + HCM_HALO_BLOCKS * 2, light, HCM_HALO_LIGHT);
 	return mesh_halo_locked(ctx, out);
+}
+
+extern "C" int hcm_mesh_halos_noalloc(hcm_ctx *ctx, const uint16_t *halo_blocks, const uint8_t *halo_light, uint32_t n,
+                                      uint32_t (*vertices)[2], uint16_t *indices, uint32_t quad_cap, hcm_mesh_rec *recs,
+                                      uint32_t rec_cap, hcm_job_result *results, void *stream) {
+	if (!ctx || !ctx->d_scratch)
+		return HCM_ERR_INVALID;
+	if (n == 0)
+		return HCM_OK;
+	if (n > kNoAllocMaxJobs || !halo_blocks || !halo_light || !vertices || !indices || !recs || !results ||
+	    quad_cap == 0 || rec_cap == 0)
+		return HCM_ERR_INVALID;
+	hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+	Counters *ctr = reinterpret_cast<Counters *>(ctx->d_scratch);
+	uint16_t *blocks = reinterpret_cast<uint16_t *>(ctx->d_scratch + kScratchBlocks);
+	uint8_t *light = ctx->d_scratch + kScratchLight;
+	HCM_HIP(launch_halo_unpack(halo_blocks, halo_light, n, blocks, light, st));
+	HCM_HIP(launch_reset(ctr, st)); // a kernel, not a memset, so that a captured graph replays it correctly
+	MeshParams p = base_params(ctx);
+	p.blocks = blocks;
+	p.light = light;
+	p.nbr = reinterpret_cast<const uint32_t *>(ctx->d_scratch + kScratchNbr);
+	p.n_jobs = n;
+	p.job_base = 0;
+	p.persistent = 0;
+	p.pool_cap = quad_cap;
+	p.ctr = ctr;
+	p.pool_verts = reinterpret_cast<uint2 *>(vertices);
+	p.pool_idx = indices;
+	p.recs = recs;
+	p.rec_cap = rec_cap;
+	p.results = results;
+	HCM_HIP(launch_mesh(p, kChunkLaunchThreads, (int)n, st));
+	return HCM_OK;
 }
 
 extern "C" int hcm_mesh_chunk(hcm_ctx *ctx, const uint16_t *const blocks[27], const uint8_t *const light[27],

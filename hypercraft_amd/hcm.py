@@ -25,6 +25,8 @@ HCM_ERR_BUSY = -6
 HALO_BLOCKS = 20 * 20 * 18  # blocks over x, z in [-2, 18), y in [-1, 17): index (x+2) + 20 ((z+2) + 20 (y+1))
 HALO_LIGHT = 18 * 18 * 18   # light over [-1, 17)^3: index (x+1) + 18 ((z+1) + 18 (y+1))
 
+NOALLOC_MAX_CHUNKS = 8  # HCM_NOALLOC_MAX_CHUNKS
+
 RUN_PERSISTENT = 0
 RUN_PER_CHUNK = 1
 LIGHT_STORE = 0    # mesh with the store's light bytes
@@ -120,6 +122,8 @@ SIGNATURES = [
     ("hcm_device_ok", ctypes.c_int, []),
     ("hcm_mesh_chunk", ctypes.c_int, [_VP, _VP, _VP, ctypes.POINTER(HostResultC)]),
     ("hcm_mesh_halo", ctypes.c_int, [_VP, _VP, _VP, ctypes.POINTER(HostResultC)]),
+    ("hcm_mesh_halos_noalloc", ctypes.c_int, [_VP, _VP, _VP, ctypes.c_uint32, _VP, _VP, ctypes.c_uint32, _VP,
+                                              ctypes.c_uint32, _VP, _VP]),
     ("hcm_mesh_chunk_relight", ctypes.c_int, [_VP, _VP, _VP, ctypes.POINTER(HostResultC), _VP]),
     ("hcm_store_create", ctypes.c_int, [_VP, ctypes.c_uint32, ctypes.POINTER(_VP)]),
     ("hcm_store_destroy", None, [_VP]),

@@ -194,6 +194,22 @@ typedef struct hcm_job_result {
 	uint32_t quad_count;
 } hcm_job_result;
 
+/* No counterpart in the reference. For callers that cannot allocate -- code under HIP graph capture,
+ * tight serving loops -- the form of hcm_mesh_halo that allocates nothing and does not synchronise: n
+ * packed halos already in device memory (halo j at halo_blocks + j HCM_HALO_BLOCKS and halo_light +
+ * j HCM_HALO_LIGHT) are meshed on `stream` (a hipStream_t; NULL: the default stream) into the caller's
+ * device buffers: quads in vertices[4 q ..]/indices[6 q ..] for q < quad_cap, mesh records in
+ * recs[0 .. rec_cap) (first_quad into those buffers), per-halo results in results[0 .. n) -- the
+ * hcm_batch_download layout. The kernels' other memory (work counters, job table, the 27-slot stores the
+ * halos are scattered into) is a fixed device array of the library, one per GPU, held until the process
+ * ends: n is at most HCM_NOALLOC_MAX_CHUNKS (HCM_ERR_INVALID beyond). Calls that overlap on one GPU share
+ * that array and race; calls on one stream do not. A halo whose output does not fit gets
+ * HCM_ERR_POOL_FULL in its result. */
+#define HCM_NOALLOC_MAX_CHUNKS 8
+int hcm_mesh_halos_noalloc(hcm_ctx *ctx, const uint16_t *halo_blocks, const uint8_t *halo_light, uint32_t n,
+                           uint32_t (*vertices)[2], uint16_t *indices, uint32_t quad_cap, hcm_mesh_rec *recs,
+                           uint32_t rec_cap, hcm_job_result *results, void *stream);
+
 typedef struct hcm_batch hcm_batch;
 
 #define HCM_RUN_PERSISTENT 0u /* one persistent launch with a work queue, replayed as a HIP graph */

@@ -27,6 +27,8 @@
 #define __forceinline__ inline
 #define __launch_bounds__(...)
 #define __shared__ static
+#define __align__(n) alignas(n)
+#define HIP_SYMBOL(x) (&(x))
 
 struct uint2 {
 	uint32_t x, y;
@@ -160,8 +162,11 @@ struct hipDeviceProp_t {
 	int major, minor;
 };
 
+extern "C" {
 hipError_t hipMalloc(void **p, size_t n);
+}
 template <typename T> inline hipError_t hipMalloc(T **p, size_t n) { return hipMalloc(reinterpret_cast<void **>(p), n); }
+extern "C" {
 hipError_t hipFree(void *p);
 hipError_t hipHostMalloc(void **p, size_t n, unsigned flags = 0);
 hipError_t hipHostFree(void *p);
@@ -193,7 +198,9 @@ hipError_t hipSetDevice(int d);
 hipError_t hipGetDeviceProperties(hipDeviceProp_t *p, int d);
 hipError_t hipDeviceSynchronize();
 hipError_t hipGetLastError();
+hipError_t hipGetSymbolAddress(void **p, const void *symbol);
 const char *hipGetErrorString(hipError_t e);
+}
 
 namespace hipcpu {
 // a launch: runs now, or is recorded when the stream is being captured

@@ -210,6 +210,8 @@ void submit(hipStream_t s, const std::function<void()> &work) {
 
 using namespace hipcpu;
 
+extern "C" {
+
 hipError_t hipMalloc(void **p, size_t n) {
 	const size_t m = (n + 255) / 256 * 256;
 	*p = std::aligned_alloc(256, m ? m : 256);
@@ -316,4 +318,10 @@ hipError_t hipGetDeviceProperties(hipDeviceProp_t *p, int) {
 }
 hipError_t hipDeviceSynchronize() { return hipSuccess; }
 hipError_t hipGetLastError() { return hipSuccess; }
+hipError_t hipGetSymbolAddress(void **p, const void *symbol) {
+	*p = const_cast<void *>(symbol);
+	return hipSuccess;
+}
 const char *hipGetErrorString(hipError_t) { return "hipcpu"; }
+
+} // extern "C"
