@@ -128,6 +128,11 @@ hipError_t launch_reset(Counters *ctr, hipStream_t stream);
 hipError_t launch_mesh(const MeshParams &p, int threads, int grid, hipStream_t stream);
 hipError_t launch_halo_unpack(const uint16_t *hblocks, const uint8_t *hlight, uint32_t n, uint16_t *blocks,
                               uint8_t *light, hipStream_t stream);
+// staged chunk k (sblocks + 4096 k, and slight + 4096 k or sheights + 256 k; the unused one null) to store
+// slot moves[i].y for k = moves[i].x, i < n; all arrays 16-byte aligned
+hipError_t launch_upload_scatter(const uint16_t *sblocks, const uint8_t *slight, const int8_t *sheights,
+                                 const uint2 *moves, uint32_t n, uint16_t *blocks, uint8_t *light, int8_t *heights,
+                                 hipStream_t stream);
 hipError_t launch_edits(uint16_t *blocks, uint8_t *light, int8_t *heights, const VoxelEdit *edits, uint32_t n,
                         hipStream_t stream);
 int mesh_lds_bytes(int threads);

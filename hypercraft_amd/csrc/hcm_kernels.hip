@@ -1329,23 +1329,6 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 #pragma unroll
 	for (int a = 0; a < 6; ++a)
 		box[0][a] = box[1][a] = a < 3 ? ~0u : 0u;
-	// custom quads, in the reference's order (cells in YZX order, faces in mesh order), per thread range
-	if (flags & kFlagCustom) {
-		uint32_t io = (uint32_t)cust_pre, it = (uint32_t)(cust_pre >> 32);
-		for (int k = 0; k < CPT; ++k) {
-			const int cell = tid * CPT + k;
-			if (!((s.custom_bits[cell >> 5] >> (cell & 31)) & 1u))
-				continue;
-			const CustomCell c = custom_cell(p, s, cell);
-			for (uint32_t j = 0; j < c.count; ++j) {
-				const uint32_t tr = (c.trans >> j) & 1u, epos = io + it, i = tr ? it++ : io++;
-				uint32_t slot, vbase, mk, mj, lo[3], hi[3];
-				quad_slot(tr, i, n0, pool_base, slot, vbase, mk, mj);
-				emit_custom(p, s, uint32_t(cell) | (j << 12), slot, vbase, lo, hi);
-				quad_done(s, tr, mk, mj, epos, lo, hi, box);
-			}
-		}
-	}
 	// regular quads, one lane per visible face (a quad per face that starts one): first the last group,
 	// whose face words are still in key, then the earlier groups with their words computed again
 	{
@@ -1387,6 +1370,41 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 				keep[j] = keep[j + 1];
 		}
 		t0 = t1;
+	}
+	// custom quads, after the regular ones (their positions in the output are fixed by the prefix sums): in
+	// windows of the face-word stage, the threads list their cells' quads (cells in YZX order, faces in
+	// mesh order, each thread its range of cells), then one lane per listed quad writes it
+	if (flags & kFlagCustom) {
+		const uint32_t n_cust = cust0 + cust1;
+#pragma unroll 1
+		for (uint32_t w0 = 0; w0 < n_cust; w0 += kListCap) {
+			__syncthreads(); // key is free: the regular quads and the previous window are written
+			const uint32_t w1 = w0 + kListCap;
+			uint32_t io = (uint32_t)cust_pre, it = (uint32_t)(cust_pre >> 32);
+			if (io + it < w1 && io + it + c_o + c_t > w0) {
+				for (int k = 0; k < CPT && io + it < w1; ++k) {
+					const int cell = tid * CPT + k;
+					if (!((s.custom_bits[cell >> 5] >> (cell & 31)) & 1u))
+						continue;
+					const CustomCell c = custom_cell(p, s, cell);
+					for (uint32_t j = 0; j < c.count; ++j) {
+						const uint32_t tr = (c.trans >> j) & 1u, g = io + it, i = tr ? it++ : io++;
+						if (g >= w0 && g < w1)
+							s.key[g - w0] = make_uint2(uint32_t(cell) | (j << 12) | (tr << 31), i);
+					}
+				}
+			}
+			__syncthreads();
+			const uint32_t n = min(n_cust - w0, (uint32_t)kListCap);
+			for (uint32_t e = tid; e < n; e += NT) {
+				const uint2 ent = s.key[e];
+				const uint32_t tr = ent.x >> 31;
+				uint32_t slot, vbase, mk, mj, lo[3], hi[3];
+				quad_slot(tr, ent.y, n0, pool_base, slot, vbase, mk, mj);
+				emit_custom(p, s, ent.x & 0xffffu, slot, vbase, lo, hi);
+				quad_done(s, tr, mk, mj, w0 + e, lo, hi, box);
+			}
+		}
 	}
 
 #pragma unroll
@@ -1540,6 +1558,35 @@ hipError_t launch_edits(uint16_t *blocks, uint8_t *light, int8_t *heights, const
 	if (n == 0)
 		return hipSuccess;
 	hipLaunchKernelGGL(edit_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, blocks, light, heights, edits, n);
+	return hipGetLastError();
+}
+
+// Copies staged chunk uploads into their store slots, one workgroup per chunk: move k copies staged
+// chunk moves[k].x to slot moves[k].y -- its 4096 blocks, and its 4096 light bytes or its 256 sunlight
+// heights, whichever staging array is given.
+__global__ void upload_scatter_kernel(const uint4 *sblocks, const uint4 *slight, const uint4 *sheights,
+                                      const uint2 *moves, uint4 *blocks, uint4 *light, uint4 *heights) {
+	const uint2 m = moves[blockIdx.x];
+	const uint32_t t = threadIdx.x;
+	constexpr uint32_t kB = HCM_CHUNK_VOXELS * 2 / 16, kL = HCM_CHUNK_VOXELS / 16, kH = 256 / 16;
+	blocks[(size_t)m.y * kB + t] = sblocks[(size_t)m.x * kB + t];
+	blocks[(size_t)m.y * kB + 256 + t] = sblocks[(size_t)m.x * kB + 256 + t];
+	if (slight)
+		light[(size_t)m.y * kL + t] = slight[(size_t)m.x * kL + t];
+	if (sheights && t < kH)
+		heights[(size_t)m.y * kH + t] = sheights[(size_t)m.x * kH + t];
+}
+static_assert(HCM_CHUNK_VOXELS * 2 / 16 == 512 && HCM_CHUNK_VOXELS / 16 == 256, "one 256-thread workgroup per chunk");
+
+hipError_t launch_upload_scatter(const uint16_t *sblocks, const uint8_t *slight, const int8_t *sheights,
+                                 const uint2 *moves, uint32_t n, uint16_t *blocks, uint8_t *light, int8_t *heights,
+                                 hipStream_t stream) {
+	if (n == 0)
+		return hipSuccess;
+	hipLaunchKernelGGL(upload_scatter_kernel, dim3(n), dim3(256), 0, stream, reinterpret_cast<const uint4 *>(sblocks),
+	                   reinterpret_cast<const uint4 *>(slight), reinterpret_cast<const uint4 *>(sheights), moves,
+	                   reinterpret_cast<uint4 *>(blocks), reinterpret_cast<uint4 *>(light),
+	                   reinterpret_cast<uint4 *>(heights));
 	return hipGetLastError();
 }
 

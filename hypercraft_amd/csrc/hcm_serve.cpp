@@ -64,7 +64,8 @@ inline int32_t floor_div16(int32_t v) { return v >= 0 ? v / 16 : -((-v + 15) / 1
 constexpr int kRev[3] = {1, -1, 0};
 inline void slot_offset(int s, int d[3]) { d[0] = kRev[s / 9], d[1] = kRev[(s / 3) % 3], d[2] = kRev[s % 3]; }
 
-constexpr uint32_t kMaxUploads = 1024;  // chunk uploads staged per batch
+constexpr uint32_t kMaxUploads = 256;   // chunk uploads staged per batch (3 MB: a burst of loads goes up in
+                                        // several batches rather than one long copy ahead of every edit)
 constexpr uint32_t kMaxEdits = 65536;   // voxel edits applied per batch
 constexpr uint64_t kMinArena = 1 << 16; // quads; more than any one chunk's meshes
 
@@ -122,6 +123,11 @@ struct BatchSlot { // device and pinned buffers of one in-flight batch
 	uint8_t *h_light = nullptr;
 	int8_t *h_heights = nullptr;
 	hcm_job_result *h_res = nullptr;
+	// the batch's chunk uploads: staged on the host, copied once, then scattered to their slots on the device
+	uint16_t *d_up_blocks = nullptr;
+	uint8_t *d_up_light = nullptr;
+	int8_t *d_up_heights = nullptr;
+	uint2 *d_moves = nullptr, *h_moves = nullptr;
 	// light rebuild (HCM_LIGHT_REBUILD): the slots the jobs read, queues, rebuilt halo light
 	uint32_t *d_rslots = nullptr, *h_rslots = nullptr;
 	Counters *d_lctr = nullptr;
@@ -443,23 +449,30 @@ struct hcm_server {
 			std::vector<uint8_t>().swap(r->light);
 			std::vector<int8_t>().swap(r->heights);
 		}
-		// a slot's data: blocks, then light bytes or sunlight heights
-		auto upload = [&](uint32_t i, hipStream_t st) -> hipError_t {
-			const uint32_t slot = uploads[i].slot;
-			hipError_t e = hipMemcpyAsync(d_blocks + slot * V, b.h_blocks + i * V, V * 2, hipMemcpyHostToDevice, st);
-			if (e == hipSuccess)
-				e = relight() ? hipMemcpyAsync(d_heights + slot * 256, b.h_heights + i * 256, 256, hipMemcpyHostToDevice, st)
-				              : hipMemcpyAsync(d_light + slot * V, b.h_light + i * V, V, hipMemcpyHostToDevice, st);
-			return e;
-		};
+		// a slot's data, blocks and light bytes or sunlight heights, goes up with the batch's others in one
+		// copy per array; first-time loads are scattered to their slots on the copy stream, reloads on the
+		// compute stream after it, in order with the earlier batches that may read their slots
+		uint32_t n_fresh = 0;
 		for (uint32_t i = 0; i < n_up; ++i)
 			if (!uploads[i].reload)
-				HCM_CHECK(upload(i, s_copy));
+				b.h_moves[n_fresh++] = make_uint2(i, uploads[i].slot);
+		for (uint32_t i = 0, k = n_fresh; i < n_up; ++i)
+			if (uploads[i].reload)
+				b.h_moves[k++] = make_uint2(i, uploads[i].slot);
+		if (n_up) {
+			HCM_CHECK(hipMemcpyAsync(b.d_up_blocks, b.h_blocks, n_up * V * 2, hipMemcpyHostToDevice, s_copy));
+			if (relight())
+				HCM_CHECK(hipMemcpyAsync(b.d_up_heights, b.h_heights, n_up * 256, hipMemcpyHostToDevice, s_copy));
+			else
+				HCM_CHECK(hipMemcpyAsync(b.d_up_light, b.h_light, n_up * V, hipMemcpyHostToDevice, s_copy));
+			HCM_CHECK(hipMemcpyAsync(b.d_moves, b.h_moves, n_up * sizeof(uint2), hipMemcpyHostToDevice, s_copy));
+		}
+		HCM_CHECK(launch_upload_scatter(b.d_up_blocks, b.d_up_light, b.d_up_heights, b.d_moves, n_fresh, d_blocks,
+		                                d_light, d_heights, s_copy));
 		HCM_CHECK(hipEventRecord(b.ev_up, s_copy));
 		HCM_CHECK(hipStreamWaitEvent(s_comp, b.ev_up, 0));
-		for (uint32_t i = 0; i < n_up; ++i)
-			if (uploads[i].reload)
-				HCM_CHECK(upload(i, s_comp));
+		HCM_CHECK(launch_upload_scatter(b.d_up_blocks, b.d_up_light, b.d_up_heights, b.d_moves + n_fresh,
+		                                n_up - n_fresh, d_blocks, d_light, d_heights, s_comp));
 		uploads.erase(uploads.begin(), uploads.begin() + n_up);
 
 		uint32_t k = 0;
@@ -741,11 +754,12 @@ struct hcm_server {
 static void free_server(hcm_server *s) {
 	for (auto &b : s->slots) {
 		for (void *p : {(void *)b.d_nbr, (void *)b.d_edits, (void *)b.d_ctr, (void *)b.d_res, (void *)b.d_recs,
-		                (void *)b.d_rslots, (void *)b.d_lctr, (void *)b.d_halo})
+		                (void *)b.d_rslots, (void *)b.d_lctr, (void *)b.d_halo, (void *)b.d_up_blocks,
+		                (void *)b.d_up_light, (void *)b.d_up_heights, (void *)b.d_moves})
 			if (p)
 				(void)hipFree(p);
 		for (void *p : {(void *)b.h_nbr, (void *)b.h_edits, (void *)b.h_blocks, (void *)b.h_light, (void *)b.h_res,
-		                (void *)b.h_recs, (void *)b.h_ctr, (void *)b.h_heights, (void *)b.h_rslots})
+		                (void *)b.h_recs, (void *)b.h_ctr, (void *)b.h_heights, (void *)b.h_rslots, (void *)b.h_moves})
 			if (p)
 				(void)hipHostFree(p);
 		for (hipEvent_t e : {b.ev_up, b.ev_k0, b.ev_k1, b.ev_done})
@@ -799,9 +813,16 @@ static int server_init(hcm_server *s) {
 		HCM_CHECK(halloc(&b.h_nbr, (size_t)c.max_batch * 27));
 		HCM_CHECK(halloc(&b.h_edits, kMaxEdits));
 		HCM_CHECK(halloc(&b.h_blocks, (size_t)kMaxUploads * HCM_CHUNK_VOXELS));
-		HCM_CHECK(halloc(&b.h_light, (size_t)kMaxUploads * HCM_CHUNK_VOXELS));
+		HCM_CHECK(dalloc(&b.d_up_blocks, (size_t)kMaxUploads * HCM_CHUNK_VOXELS));
+		HCM_CHECK(halloc(&b.h_moves, kMaxUploads));
+		HCM_CHECK(dalloc(&b.d_moves, kMaxUploads));
+		if (!s->relight()) {
+			HCM_CHECK(halloc(&b.h_light, (size_t)kMaxUploads * HCM_CHUNK_VOXELS));
+			HCM_CHECK(dalloc(&b.d_up_light, (size_t)kMaxUploads * HCM_CHUNK_VOXELS));
+		}
 		if (s->relight()) {
 			HCM_CHECK(halloc(&b.h_heights, (size_t)kMaxUploads * 256));
+			HCM_CHECK(dalloc(&b.d_up_heights, (size_t)kMaxUploads * 256));
 			HCM_CHECK(dalloc(&b.d_rslots, (size_t)c.max_batch * 27));
 			HCM_CHECK(halloc(&b.h_rslots, (size_t)c.max_batch * 27));
 			HCM_CHECK(dalloc(&b.d_lctr, 1));

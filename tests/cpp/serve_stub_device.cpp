@@ -82,6 +82,20 @@ hipError_t launch_edits(uint16_t *blocks, uint8_t *light, int8_t *heights, const
 	return hipSuccess;
 }
 
+hipError_t launch_upload_scatter(const uint16_t *sblocks, const uint8_t *slight, const int8_t *sheights,
+                                 const uint2 *moves, uint32_t n, uint16_t *blocks, uint8_t *light, int8_t *heights,
+                                 hipStream_t) {
+	for (uint32_t i = 0; i < n; ++i) {
+		const size_t k = moves[i].x, slot = moves[i].y;
+		std::memcpy(blocks + slot * HCM_CHUNK_VOXELS, sblocks + k * HCM_CHUNK_VOXELS, HCM_CHUNK_VOXELS * 2);
+		if (slight)
+			std::memcpy(light + slot * HCM_CHUNK_VOXELS, slight + k * HCM_CHUNK_VOXELS, HCM_CHUNK_VOXELS);
+		if (sheights)
+			std::memcpy(heights + slot * 256, sheights + k * 256, 256);
+	}
+	return hipSuccess;
+}
+
 hipError_t launch_light_prep(const LightParams &, hipStream_t) { return hipSuccess; }
 hipError_t launch_light(const LightParams &p, int, hipStream_t) {
 	std::memset(p.halo_light, 0, (size_t)p.n_jobs * kLightJobBytes);
