@@ -158,11 +158,14 @@ def workloads_slice(blocks, light, nbr, j):
     return [blocks[s] for s in nbr[j]], [light[s] for s in nbr[j]]
 
 
-def relight_leg(ctx, hcm, workloads, meshdigest, steps, cpu_threads, with_cpu):
+def relight_leg(ctx, hcm, workloads, meshdigest, steps, cpu_threads, with_cpu, order="linear"):
     """The whole mesh task on the lit cave region (the cave region with glowstone in 0.05% of its air and
     sunlight heights from its columns): light rebuilt from blocks and heights (ChunkMeshTask.cpp:48-111),
-    then meshing, one captured graph per step. Bit-exact against the reference's mesh digests."""
+    then meshing, one captured graph per step, with the job order the tuner chose for the cave batch
+    (both kernels cut the job list into per-XCD parts). Bit-exact against the reference's mesh digests."""
     import hashlib
+
+    from hypercraft_amd import tune
 
     blocks, heights, nbr = workloads.lit_cave_region()
     golden = np.load(os.path.join(ROOT, "tests", "golden", "light.npz"))
@@ -170,29 +173,38 @@ def relight_leg(ctx, hcm, workloads, meshdigest, steps, cpu_threads, with_cpu):
     store = hcm.Store(ctx, blocks.shape[0])
     store.upload(blocks, np.zeros(blocks.shape, np.uint8))
     store.upload_sunlight(heights)
-    batch = hcm.Batch(ctx, store, nbr)
+    perm = np.arange(n)
+    if order != "linear":
+        cost = None
+        if "_" in order:  # the cost orders: the jobs' quads from one run in the region's order
+            b0 = hcm.Batch(ctx, store, nbr)
+            b0.set_light(hcm.LIGHT_REBUILD)
+            b0.run()
+            b0.sync()
+            cost = tune.job_costs(b0)
+            b0.close()
+        perm = tune.job_order(order, workloads.region_job_positions(), cost)
+    batch = hcm.Batch(ctx, store, nbr[perm])  # job k is the region's job perm[k]
     batch.set_light(hcm.LIGHT_REBUILD)
     for _ in range(3):
         batch.run()
     batch.sync()
-    ms = []
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        batch.run()
-        ms.append(batch.sync())
+    t0 = time.perf_counter()  # steps back to back
+    batch.run_repeat(steps)
+    ms = [batch.sync() / steps]
     dt = time.perf_counter() - t0
     quads, meshes = batch.totals()
     out = batch.download()
-    bad = sum(meshdigest.digest(hcm.job_meshes(j, *out)) != golden["lit_cave_region_mesh_digest"][j].tobytes()
+    bad = sum(meshdigest.digest(hcm.job_meshes(j, *out)) != golden["lit_cave_region_mesh_digest"][perm[j]].tobytes()
               for j in range(n))
     bad_light = sum(hashlib.sha256(batch.fetch_light(j).tobytes()).digest() !=
-                    golden["lit_cave_region_halo_digest"][j].tobytes() for j in range(0, n, 16))
+                    golden["lit_cave_region_halo_digest"][perm[j]].tobytes() for j in range(0, n, 16))
     batch.close()
     store.close()
     res = {"workload": "lit cave region: 4096 chunks, glowstone in 0.05% of the air, sunlight heights from the "
                        "columns; light rebuild + meshing per step",
            "chunks_per_s": round(n * steps / dt, 1), "ms_per_step": round(dt * 1e3 / steps, 4),
-           "kernel_ms": round(float(np.mean(ms)), 4), "quads_per_step": int(quads),
+           "kernel_ms": round(float(np.mean(ms)), 4), "quads_per_step": int(quads), "job_order": order,
            "bit_exact": bool(bad == 0 and bad_light == 0 and (out[0]["status"] == 0).all())}
     if with_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -389,6 +401,8 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--tuned", default="", metavar="THREADS,WG_PER_CU,ORDER",
+                    help="use this tuning instead of sweeping (e.g. 256,3,morton_cost), for profiling runs")
     ap.add_argument("--no-relight", action="store_true")
     ap.add_argument("--no-dropin", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -419,10 +433,16 @@ def main():
     tuning = ctx.get_tuning()
     tuning["order"] = "linear"
     perm = np.arange(n)
-    if not args.no_tune:
+    if args.tuned:
+        t, w, o = args.tuned.split(",")
+        log(f"tuning given: {t} threads, {w} workgroups per CU, order {o}")
+        tuned = tune.autotune(ctx, store, nbr, workload="cave_region", positions=positions,
+                              want=golden["cave_region_digest"], reps=1, candidates=[(int(t), int(w), o)], log=log)
+    elif not args.no_tune:
         log("autotune")
         tuned = tune.autotune(ctx, store, nbr, workload="cave_region", positions=positions,
                               want=golden["cave_region_digest"], reps=3, log=log)
+    if args.tuned or not args.no_tune:
         tuning = {"threads": tuned["threads"], "wg_per_cu": tuned["wg_per_cu"], "order": tuned["order"]}
         perm = tuned["perm"]
     batch = hcm.Batch(ctx, store, nbr[perm])  # job k is the region's job perm[k]
@@ -433,11 +453,12 @@ def main():
     batch.sync()
 
     log(f"timing {args.steps} steps")
-    kernel_ms = []
+    # the timed region: K graph replays back to back, bracketed by device synchronisation on the host
+    # and by HIP events on the context's stream (the kernel time per launch is their time over K)
+    batch.sync()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        batch.run()
-        kernel_ms.append(batch.sync())
+    batch.run_repeat(args.steps)
+    kernel_ms = [batch.sync() / args.steps]
     dt = time.perf_counter() - t0
     quads, meshes = batch.totals()
     out = batch.download()
@@ -507,7 +528,7 @@ def main():
     if not args.no_relight:
         log("light rebuild + meshing")
         res["relight"] = relight_leg(ctx, hcm, workloads, meshdigest, 20, min(host_cpus()[0], CPU_SHARE),
-                                     not args.no_cpu)
+                                     not args.no_cpu, tuning.get("order", "linear"))
     if not args.no_dropin:
         log("drop-in call latency")
         res["dropin"] = dropin_leg(ctx, hcm, blocks, light, nbr, golden["cave_region_digest"])

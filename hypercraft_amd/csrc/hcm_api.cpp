@@ -988,7 +988,8 @@ static hipError_t enqueue_light(hcm_batch *b, hipStream_t st) {
 	return e;
 }
 
-extern "C" int hcm_batch_run(hcm_batch *b, uint32_t mode) {
+// `times` graph replays back to back between the timing events (persistent runs after the first)
+static int batch_run(hcm_batch *b, uint32_t mode, uint32_t times) {
 	if (!b || (mode != HCM_RUN_PERSISTENT && mode != HCM_RUN_PER_CHUNK))
 		return HCM_ERR_INVALID;
 	hcm_ctx *c = b->ctx;
@@ -1037,7 +1038,8 @@ extern "C" int hcm_batch_run(hcm_batch *b, uint32_t mode) {
 		if (!b->ctr_clean) // after per-chunk launches
 			HCM_HIP(launch_reset(b->d_ctr, st));
 		HCM_HIP(hipEventRecord(b->ev0, st));
-		HCM_HIP(hipGraphLaunch(b->graph_exec, st));
+		for (uint32_t k = 0; k < times; ++k)
+			HCM_HIP(hipGraphLaunch(b->graph_exec, st));
 		HCM_HIP(hipEventRecord(b->ev1, st));
 		b->ctr_clean = true;
 	} else {
@@ -1058,6 +1060,19 @@ extern "C" int hcm_batch_run(hcm_batch *b, uint32_t mode) {
 	b->light_ran = b->light_mode == HCM_LIGHT_REBUILD;
 	b->last_mode = mode;
 	return HCM_OK;
+}
+
+extern "C" int hcm_batch_run(hcm_batch *b, uint32_t mode) { return batch_run(b, mode, 1); }
+
+extern "C" int hcm_batch_run_repeat(hcm_batch *b, uint32_t times) {
+	if (!b || times == 0)
+		return HCM_ERR_INVALID;
+	if (!b->launched) { // the first persistent run is a direct launch
+		const int r = batch_run(b, HCM_RUN_PERSISTENT, 1);
+		if (r != HCM_OK || --times == 0)
+			return r;
+	}
+	return batch_run(b, HCM_RUN_PERSISTENT, times);
 }
 
 extern "C" int hcm_batch_sync(hcm_batch *b, float *kernel_ms) {

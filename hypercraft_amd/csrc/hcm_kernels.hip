@@ -125,7 +125,7 @@ template <int NT> struct Smem {
 	uint32_t tmpl_id[256];     // voxel-word template by block id when the meta byte does not change it, else kTmplMeta
 	uint32_t aabb[2][kMaxMeshPerPass][6];
 	uint32_t events[16][2];
-	uint64_t scan[NT / 64 + 1];
+	uint64_t scan2[NT / 64][2];
 	uint32_t slots[27];
 	uint32_t job, flags, n_events, status;
 	uint32_t wave_ref[NT / 64];
@@ -268,32 +268,33 @@ template <bool kMax> __device__ __forceinline__ uint32_t wave_fold(uint32_t v) {
 // ------------------------------------------------------------------------------------------------
 // block-wide exclusive scan of 64-bit values (two 32-bit counters packed lo | hi << 32)
 // ------------------------------------------------------------------------------------------------
-template <int NT> __device__ __forceinline__ uint64_t block_excl_scan(Smem<NT> &s, uint64_t v, uint64_t *total) {
+// Exclusive prefix sums of two values over the workgroup's threads in one pass (one barrier): returns
+// each thread's prefixes in a and b and the totals in ta and tb. Every thread reads s.scan2 after the
+// barrier; the next write to it is a later chunk's, after several more barriers.
+template <int NT>
+__device__ __forceinline__ void block_excl_scan2(Smem<NT> &s, uint64_t &a, uint64_t &b, uint64_t &ta, uint64_t &tb) {
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-	uint64_t inc = v;
+	uint64_t ia = a, ib = b;
 #pragma unroll
 	for (int d = 1; d < 64; d <<= 1) {
-		const uint32_t lo = __shfl_up((uint32_t)inc, d), hi = __shfl_up((uint32_t)(inc >> 32), d);
-		if (lane >= d)
-			inc += (uint64_t)lo | ((uint64_t)hi << 32);
+		const uint32_t al = __shfl_up((uint32_t)ia, d), ah = __shfl_up((uint32_t)(ia >> 32), d);
+		const uint32_t bl = __shfl_up((uint32_t)ib, d), bh = __shfl_up((uint32_t)(ib >> 32), d);
+		if (lane >= d) {
+			ia += (uint64_t)al | ((uint64_t)ah << 32);
+			ib += (uint64_t)bl | ((uint64_t)bh << 32);
+		}
 	}
 	if (lane == 63)
-		s.scan[wave] = inc;
+		s.scan2[wave][0] = ia, s.scan2[wave][1] = ib;
 	__syncthreads();
-	if (threadIdx.x == 0) {
-		uint64_t run = 0;
-		for (int w = 0; w < NT / 64; ++w) {
-			const uint64_t t = s.scan[w];
-			s.scan[w] = run;
-			run += t;
-		}
-		s.scan[NT / 64] = run;
+	uint64_t ra = 0, rb = 0, sa = 0, sb = 0;
+#pragma unroll
+	for (int w = 0; w < NT / 64; ++w) {
+		const uint64_t xa = s.scan2[w][0], xb = s.scan2[w][1];
+		ra += w < wave ? xa : 0, rb += w < wave ? xb : 0;
+		sa += xa, sb += xb;
 	}
-	__syncthreads();
-	const uint64_t r = s.scan[wave] + inc - v;
-	*total = s.scan[NT / 64];
-	__syncthreads();
-	return r;
+	a = ra + ia - a, b = rb + ib - b, ta = sa, tb = sb;
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -881,6 +882,8 @@ __device__ __forceinline__ void greedy_count(Smem<NT> &s, int t, bool active, ui
 // The end of the longest run of tasks from t0 whose visible faces fit the face-word stage (one task, at
 // most 256 faces, always fits). Thread 0 searches; a barrier publishes the result.
 template <int NT> __device__ __forceinline__ int group_end(Smem<NT> &s, int t0) {
+	if (s.fpre[kTasks] - s.fpre[t0] <= (uint32_t)kListCap) // the rest fits (the usual case): every thread knows
+		return kTasks;
 	if (threadIdx.x == 0) {
 		int lo = t0 + 1, hi = kTasks; // largest t1 with fpre[t1] - fpre[t0] <= kListCap
 		while (lo < hi) {
@@ -1286,9 +1289,9 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 	}
 	// 5. block-wide prefix sums and allocation: the chunk's output is [pool_base, +n0) opaque and
 	// [+n0, +n0+n1) transparent; custom quads come first in each pass, then the tasks' quads in task order
+	uint64_t cust_pre = (uint64_t)c_o | ((uint64_t)c_t << 32), task_pre = (uint64_t)q_o | ((uint64_t)q_t << 32);
 	uint64_t cust_total, task_total;
-	const uint64_t cust_pre = block_excl_scan(s, (uint64_t)c_o | ((uint64_t)c_t << 32), &cust_total);
-	const uint64_t task_pre = block_excl_scan(s, (uint64_t)q_o | ((uint64_t)q_t << 32), &task_total);
+	block_excl_scan2(s, cust_pre, task_pre, cust_total, task_total);
 	const uint32_t cust0 = (uint32_t)cust_total, cust1 = (uint32_t)(cust_total >> 32);
 	const uint32_t n0 = cust0 + (uint32_t)task_total, n1 = cust1 + (uint32_t)(task_total >> 32);
 	const uint32_t total = n0 + n1;

@@ -161,28 +161,39 @@ __device__ __forceinline__ void light_slots(const LightParams &p, LightSmem &s, 
 		s.levels = lv & ~1u;
 }
 
-// One job; s.slots and s.levels are in place (a barrier after them).
+// One job; s.slots and s.levels are in place (a barrier after them). Every window row is read from the
+// packed planes once: its light pass and sunlit cells into LDS, its level-15 torchlight sources into
+// the reading thread's registers until the torchlight channel starts.
 template <int NT> __device__ __forceinline__ void light_job(const LightParams &p, LightSmem &s, uint32_t job) {
+	static_assert(NT >= kHRows, "one halo x-row per thread");
+	constexpr int kRowsPerThread = (kWRows + NT - 1) / NT;
 	const int tid = threadIdx.x;
 	const bool torch = s.levels != 0, torch15 = (s.levels >> 15) & 1u;
-	for (int r = tid; r < kWRows; r += NT) {
-		const int wy = r / kW, wz = r % kW;
-		const RowSrc src = row_src(p, s, wy, wz);
-		if (unsigned(wy - 1) < unsigned(kIn) && unsigned(wz - 1) < unsigned(kIn))
-			s.pass[(wz - 1) + kIn * (wy - 1)] = plane_row(src, 0);
-		s.g[0][r] = plane_row(src, 1); // sunlight: 15 at and above the sunlight height (Chunk.hpp:73-79)
+	uint64_t src15[kRowsPerThread];
+#pragma unroll
+	for (int k = 0; k < kRowsPerThread; ++k) {
+		const int r = tid + k * NT;
+		src15[k] = 0;
+		if (r < kWRows) {
+			const int wy = r / kW, wz = r % kW;
+			const RowSrc src = row_src(p, s, wy, wz);
+			if (unsigned(wy - 1) < unsigned(kIn) && unsigned(wz - 1) < unsigned(kIn))
+				s.pass[(wz - 1) + kIn * (wy - 1)] = plane_row(src, 0);
+			s.g[0][r] = plane_row(src, 1); // sunlight: 15 at and above the sunlight height (Chunk.hpp:73-79)
+			if (torch15)
+				src15[k] = plane_row(src, 6);
+		}
 	}
 	__syncthreads();
-	// light bytes sun | torch << 4 (block/include/block/Light.hpp), one 20-byte record per halo x-row:
-	// the sunlight nibbles first, then the torchlight nibbles into the same records (each thread rewrites
-	// only the records it wrote)
-	uint32_t *out = reinterpret_cast<uint32_t *>(p.halo_light + (size_t)job * kLightJobBytes);
-	auto put = [&](int shift) {
-		for (int hr = tid; hr < kHRows; hr += NT) {
+	// light bytes sun | torch << 4 (block/include/block/Light.hpp), one 20-byte record per halo x-row,
+	// written once: a thread's halo row's sunlight nibbles wait in its registers for the torchlight ones
+	const int hr = tid;
+	uint32_t w[5] = {0, 0, 0, 0, 0};
+	auto nibbles = [&](int shift) {
+		if (hr < kHRows) {
 			const uint64_t l012 = s.lv012[hr];
 			const uint32_t pl[4] = {uint32_t(l012) & 0x3ffffu, uint32_t(l012 >> 18) & 0x3ffffu,
 			                        uint32_t(l012 >> 36) & 0x3ffffu, s.lv3[hr]};
-			uint32_t w[5] = {0, 0, 0, 0, 0};
 #pragma unroll
 			for (int x = 0; x < kH; ++x) {
 				uint32_t v = 0;
@@ -191,20 +202,25 @@ template <int NT> __device__ __forceinline__ void light_job(const LightParams &p
 					v |= ((pl[b] >> x) & 1u) << b;
 				w[x >> 2] |= v << (8 * (x & 3) + shift);
 			}
-			uint32_t *o = out + hr * (kLightRowBytes / 4);
-#pragma unroll
-			for (int q = 0; q < 5; ++q)
-				o[q] = shift ? (o[q] | w[q]) : w[q];
 		}
 	};
 	propagate<NT>(p, s, false);
-	put(0);
+	nibbles(0);
 	if (torch) {
-		for (int r = tid; r < kWRows; r += NT)
-			s.g[0][r] = torch15 ? plane_row(row_src(p, s, r / kW, r % kW), 6) : 0ull;
+		__syncthreads(); // every halo row's sunlight level is read
+#pragma unroll
+		for (int k = 0; k < kRowsPerThread; ++k)
+			if (tid + k * NT < kWRows)
+				s.g[0][tid + k * NT] = src15[k];
 		__syncthreads();
 		propagate<NT>(p, s, true);
-		put(4);
+		nibbles(4);
+	}
+	if (hr < kHRows) {
+		uint32_t *o = reinterpret_cast<uint32_t *>(p.halo_light + (size_t)job * kLightJobBytes) + hr * (kLightRowBytes / 4);
+#pragma unroll
+		for (int q = 0; q < 5; ++q)
+			o[q] = w[q];
 	}
 }
 

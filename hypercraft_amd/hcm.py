@@ -133,6 +133,7 @@ SIGNATURES = [
     ("hcm_batch_create", ctypes.c_int, [_VP, _VP, _VP, ctypes.c_uint32, ctypes.POINTER(_VP)]),
     ("hcm_batch_destroy", None, [_VP]),
     ("hcm_batch_run", ctypes.c_int, [_VP, ctypes.c_uint32]),
+    ("hcm_batch_run_repeat", ctypes.c_int, [_VP, ctypes.c_uint32]),
     ("hcm_batch_sync", ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_float)]),
     ("hcm_batch_results", ctypes.c_int, [_VP, _VP]),
     ("hcm_batch_fetch", ctypes.c_int, [_VP, ctypes.c_uint32, ctypes.POINTER(HostResultC)]),
@@ -378,6 +379,10 @@ class Batch:
 
     def run(self, mode=RUN_PERSISTENT):
         check("hcm_batch_run", lib().hcm_batch_run(self._ptr, mode))
+
+    def run_repeat(self, times):
+        """`times` persistent passes back to back; sync() then times all of them together."""
+        check("hcm_batch_run_repeat", lib().hcm_batch_run_repeat(self._ptr, times))
 
     def set_light(self, mode):
         """LIGHT_STORE: mesh with the store's light bytes; LIGHT_REBUILD: rebuild light first."""

@@ -20,7 +20,7 @@ import numpy as np
 
 THREADS = (128, 256, 512)
 WG_PER_CU = (0, 1, 2, 3, 4)  # 0: as many as the LDS footprint allows
-ORDERS = ("linear", "morton", "linear_cost", "morton_cost")
+ORDERS = ("linear", "morton", "linear_cost", "morton_cost", "morton_zerolast")
 QUEUES = 8  # the kernel's job-list parts, one per XCD (hcm_internal.h kQueues)
 
 
@@ -29,7 +29,8 @@ def job_order(name, positions, cost=None):
     3-D Z-order curve of their chunk positions, so each eighth of the list (one XCD's part) is a compact
     block of space and consecutive jobs are neighbours. "<order>_cost" keeps each eighth of <order> and
     sorts the jobs inside it by descending cost (per job, in the caller's order; ties keep <order>), so
-    that each part ends with its cheapest jobs."""
+    that each part ends with its cheapest jobs; "<order>_zerolast" only moves the jobs of cost 0 to the
+    end of their eighth, keeping the others in <order> (and their neighbours' halos close in time)."""
     p = np.asarray(positions, np.int64)
     base, _, by_cost = name.partition("_")
     if base == "linear":
@@ -44,8 +45,9 @@ def job_order(name, positions, cost=None):
     if by_cost:
         n = perm.shape[0]
         c = np.asarray(cost, np.float64)[perm]
-        parts = [perm[q * n // QUEUES:(q + 1) * n // QUEUES][np.argsort(-c[q * n // QUEUES:(q + 1) * n // QUEUES],
-                                                                      kind="stable")] for q in range(QUEUES)]
+        key = -c if by_cost == "cost" else (c == 0).astype(np.float64)
+        parts = [perm[q * n // QUEUES:(q + 1) * n // QUEUES][np.argsort(key[q * n // QUEUES:(q + 1) * n // QUEUES],
+                                                                        kind="stable")] for q in range(QUEUES)]
         perm = np.concatenate(parts)
     return perm
 

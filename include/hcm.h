@@ -221,6 +221,11 @@ int hcm_batch_create(hcm_ctx *ctx, hcm_store *store, const uint32_t *nbr, uint32
 void hcm_batch_destroy(hcm_batch *batch);
 /* Enqueues one meshing pass over every job (resets the pool first). Asynchronous. */
 int hcm_batch_run(hcm_batch *batch, uint32_t mode);
+/* Enqueues `times` persistent passes back to back, as many calls of hcm_batch_run(batch,
+ * HCM_RUN_PERSISTENT) would (each pass's output replaces the previous one's); hcm_batch_sync then gives
+ * the HIP-event time of all of them but a batch's first pass, which is launched on its own. A throughput
+ * form for repeated meshing of resident chunks; no counterpart in the reference. Asynchronous. */
+int hcm_batch_run_repeat(hcm_batch *batch, uint32_t times);
 /* Waits for the last run; *kernel_ms receives the HIP-event time of its kernels (with HCM_LIGHT_REBUILD
  * the light rebuild and the meshing). */
 int hcm_batch_sync(hcm_batch *batch, float *kernel_ms);

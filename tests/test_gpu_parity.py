@@ -124,6 +124,28 @@ def test_batched_persistent(ctx, golden, name):
     assert_digests(name, digests_of(out), golden[name + "_digest"])
 
 
+def test_repeated_runs_back_to_back(ctx, golden):
+    """hcm_batch_run_repeat: three passes enqueued at once on a fresh batch (its first pass launched on its
+    own, then two graph replays) and then five more leave the reference's meshes, and the event time of
+    the replays is positive."""
+    name = "terrain_plants_region"
+    blocks, light, nbr = gset(name)
+    store = hcm.Store(ctx, blocks.shape[0])
+    store.upload(blocks, light)
+    batch = hcm.Batch(ctx, store, nbr)
+    try:
+        for times in (3, 5):
+            say(f"repeated runs: {name}, {times} passes")
+            batch.run_repeat(times)
+            assert batch.sync() > 0
+            out = batch.download()
+            assert (out[0]["status"] == 0).all()
+            assert_digests(name, digests_of(out), golden[name + "_digest"])
+    finally:
+        batch.close()
+        store.close()
+
+
 def test_faceless_chunks(ctx, oracle):
     """The early exits (opaque cubes only, no chunk voxel with a face, no visible face) and the single
     voxels that defeat them: batched and plain results equal the oracle's."""
