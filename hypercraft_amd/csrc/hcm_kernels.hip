@@ -127,9 +127,9 @@ template <int NT> struct Smem {
 	uint32_t events[16][2];
 	uint64_t scan2[NT / 64][2];
 	uint32_t slots[27];
-	uint32_t job, flags, n_events, status;
+	uint32_t job, flags, n_events, n_link;
 	uint32_t wave_ref[NT / 64];
-	uint32_t pool_base, mesh_base, n_mesh0, n_mesh1, group_end;
+	uint32_t pool_base, mesh_base, group_end;
 };
 
 constexpr uint32_t kFlagNonUniform = 1u, kFlagCustom = 2u, kFlagFlow = 4u;
@@ -996,7 +996,7 @@ template <int NT> __device__ __forceinline__ void chunk_init(Smem<NT> &s) {
 		a[tid] = (tid % 6) < 3 ? 0xffffffffu : 0u; // AABB<uint32_t>{}: min = max(), max = min() (dep/AABB.hpp:11)
 	}
 	if (tid == 0)
-		s.n_events = 0, s.flags = 0;
+		s.n_events = 0, s.flags = 0, s.n_link = 0;
 }
 
 // One job; s.slots and the state of chunk_init are in place (a barrier after them).
@@ -1244,17 +1244,14 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 	// results follow from its masks directly. The others are listed and merged by one lane each, in as few
 	// wavefronts as they fill. The list passes through qpre_t and the counts through qpre_o (o | t << 16)
 	// before either holds its own values.
-	if (tid == 0)
-		s.group_end = 0;
-	__syncthreads();
-	if (tid < kTasks) {
+	if (tid < kTasks) { // the link masks are complete: the group loop ends with a barrier
 		const int t = tid;
 		uint64_t any = 0;
 #pragma unroll
 		for (int q = 0; q < 4; ++q)
 			any |= reinterpret_cast<const uint64_t *>(s.eh[t])[q] | reinterpret_cast<const uint64_t *>(s.ev[t])[q];
 		if (any) {
-			s.qpre_t[atomicAdd(&s.group_end, 1u)] = uint32_t(t);
+			s.qpre_t[atomicAdd(&s.n_link, 1u)] = uint32_t(t);
 		} else {
 			uint32_t no = 0, nt = 0;
 #pragma unroll
@@ -1272,7 +1269,7 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 	}
 	__syncthreads();
 	{
-		const uint32_t n_link = s.group_end;
+		const uint32_t n_link = s.n_link;
 		if (wave * 64 < (int)n_link) {
 			const bool active = tid < (int)n_link;
 			const int t = active ? (int)s.qpre_t[tid] : 0;
@@ -1304,27 +1301,23 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 		s.qpre_o[tid] = cust0 + (uint32_t)task_pre;
 		s.qpre_t[tid] = cust1 + (uint32_t)(task_pre >> 32);
 	}
-	if (tid == 0) {
-		const uint32_t nm0 = (n0 + kSplitQuads - 1) / kSplitQuads, nm1 = (n1 + kSplitQuads - 1) / kSplitQuads;
-		uint32_t status = HCM_OK;
-		const uint32_t pbase = atomicAdd(&p.ctr->pool_cursor, total);
-		if ((uint64_t)pbase + total > p.pool_cap)
-			status = (uint32_t)HCM_ERR_POOL_FULL;
-		const uint32_t mbase = atomicAdd(&p.ctr->mesh_cursor, nm0 + nm1);
-		if ((uint64_t)mbase + nm0 + nm1 > p.rec_cap)
-			status = (uint32_t)HCM_ERR_POOL_FULL;
-		s.status = status;
-		s.pool_base = pbase;
-		s.mesh_base = mbase;
-		s.n_mesh0 = nm0;
-		s.n_mesh1 = nm1;
-		if (status != HCM_OK)
-			p.results[job] = hcm_job_result{(int32_t)status, 0u, 0u, total};
+	// allocation: the quad range and the mesh records, one atomic each, issued by two lanes of one
+	// instruction so that both are in flight together
+	const uint32_t nm0 = (n0 + kSplitQuads - 1) / kSplitQuads, nm1 = (n1 + kSplitQuads - 1) / kSplitQuads;
+	if (tid < 2) {
+		const uint32_t base = atomicAdd(tid == 0 ? &p.ctr->pool_cursor : &p.ctr->mesh_cursor, tid == 0 ? total : nm0 + nm1);
+		if (tid == 0)
+			s.pool_base = base;
+		else
+			s.mesh_base = base;
 	}
 	__syncthreads();
-	if (s.status != HCM_OK)
-		return;
 	const uint32_t pool_base = s.pool_base;
+	if ((uint64_t)pool_base + total > p.pool_cap || (uint64_t)s.mesh_base + nm0 + nm1 > p.rec_cap) {
+		if (tid == 0)
+			p.results[job] = hcm_job_result{(int32_t)HCM_ERR_POOL_FULL, 0u, 0u, total};
+		return;
+	}
 
 	// 6. emission. AABB<uint32_t>::Merge (dep/AABB.hpp:52-59) of the first mesh of each pass per lane,
 	// reduced at the end; later meshes of a pass (past 16,383 quads) by LDS atomics
@@ -1432,7 +1425,7 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 				s.events[b][0] = s.events[b - 1][0], s.events[b][1] = s.events[b - 1][1];
 				s.events[b - 1][0] = e0, s.events[b - 1][1] = e1;
 			}
-		const uint32_t nm0 = s.n_mesh0, nm1 = s.n_mesh1, mbase = s.mesh_base;
+		const uint32_t mbase = s.mesh_base;
 		uint32_t m = 0;
 		auto put = [&](uint32_t pass, uint32_t k) {
 			const uint32_t n_p = pass ? n1 : n0;

@@ -14,6 +14,11 @@ def test_orders_are_permutations_with_the_base_parts():
     for name in tune.ORDERS:
         perm = tune.job_order(name, pos, cost)
         assert sorted(perm.tolist()) == list(range(n)), name
+        if name == "dealt_cost":  # every part gets every eighth job of the descending order
+            srt = np.argsort(-cost, kind="stable")
+            for q in range(tune.QUEUES):
+                assert perm[q * n // tune.QUEUES:(q + 1) * n // tune.QUEUES].tolist() == srt[q::tune.QUEUES].tolist()
+            continue
         base = tune.job_order(name.split("_")[0], pos)
         for q in range(tune.QUEUES):
             part = slice(q * n // tune.QUEUES, (q + 1) * n // tune.QUEUES)
