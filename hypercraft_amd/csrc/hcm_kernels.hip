@@ -538,16 +538,16 @@ __device__ __forceinline__ void emit_quad(const MeshParams &p, const Smem<NT> &s
 		du[a] = height_on_u ? on_u * h : on_v * w;
 		dv[a] = height_on_u ? on_v * w : on_u * h;
 	}
-	uint32_t w0[4], w1[4];
+	// word 0 packs x, y, z (1/16 units, at most 256) in 10-bit fields: the corners are the origin's word
+	// plus the packed quad vectors, with no carry between fields
+	const auto pack = [](uint32_t x, uint32_t y, uint32_t z) { return (x << 4) | (y << 14) | (z << 24); };
+	const uint32_t base = pack(o[0], o[1], o[2]) | (uint32_t(A) << 30);
+	const uint32_t pu = pack(du[0], du[1], du[2]), pv = pack(dv[0], dv[1], dv[2]);
+	const uint32_t w0[4] = {base, base + pu, base + pu + pv, base + pv};
+	uint32_t w1[4];
 #pragma unroll
-	for (int k = 0; k < 4; ++k) {
-		const int ku = (k == 1 || k == 2), kv = (k == 2 || k == 3);
-		const uint32_t x = uint32_t(o[0] + ku * du[0] + kv * dv[0]) << 4;
-		const uint32_t y = uint32_t(o[1] + ku * du[1] + kv * dv[1]) << 4;
-		const uint32_t z = uint32_t(o[2] + ku * du[2] + kv * dv[2]) << 4;
-		w0[k] = x | (y << 10) | (z << 20) | (uint32_t(A) << 30);
+	for (int k = 0; k < 4; ++k)
 		w1[k] = vword1(tex, uint32_t(face), l4_ao(l, k), l4_sun(l, k), l4_torch(l, k));
-	}
 	put_quad(p, slot, w0, w1, vbase, l4_flip(l));
 #pragma unroll
 	for (int a = 0; a < 3; ++a) {

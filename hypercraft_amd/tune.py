@@ -20,7 +20,7 @@ import numpy as np
 
 THREADS = (128, 256, 512)
 WG_PER_CU = (0, 1, 2, 3, 4)  # 0: as many as the LDS footprint allows
-ORDERS = ("linear", "morton", "linear_cost", "morton_cost", "morton_zerolast", "dealt_cost")
+ORDERS = ("linear", "morton", "linear_cost", "morton_cost")
 QUEUES = 8  # the kernel's job-list parts, one per XCD (hcm_internal.h kQueues)
 
 
@@ -29,21 +29,8 @@ def job_order(name, positions, cost=None):
     3-D Z-order curve of their chunk positions, so each eighth of the list (one XCD's part) is a compact
     block of space and consecutive jobs are neighbours. "<order>_cost" keeps each eighth of <order> and
     sorts the jobs inside it by descending cost (per job, in the caller's order; ties keep <order>), so
-    that each part ends with its cheapest jobs; "<order>_zerolast" only moves the jobs of cost 0 to the
-    end of their eighth, keeping the others in <order> (and their neighbours' halos close in time).
-    "dealt_cost" sorts all jobs by descending cost and deals them to the eighths in turn, so that every
-    part holds an equal share of the work, most costly first (at the price of the parts' locality)."""
+    that each part ends with its cheapest jobs."""
     p = np.asarray(positions, np.int64)
-    if name == "dealt_cost":
-        n = p.shape[0]
-        srt = np.argsort(-np.asarray(cost, np.float64), kind="stable")
-        parts = [srt[q::QUEUES] for q in range(QUEUES)]
-        # the kernel's part q is [q n / 8, (q + 1) n / 8): trim or pad each dealt share to that length
-        out = np.concatenate(parts)
-        bounds = [q * n // QUEUES for q in range(QUEUES + 1)]
-        if all(len(parts[q]) == bounds[q + 1] - bounds[q] for q in range(QUEUES)):
-            return out
-        return srt  # uneven part lengths: the plain descending order
     base, _, by_cost = name.partition("_")
     if base == "linear":
         perm = np.arange(p.shape[0])
@@ -57,9 +44,8 @@ def job_order(name, positions, cost=None):
     if by_cost:
         n = perm.shape[0]
         c = np.asarray(cost, np.float64)[perm]
-        key = -c if by_cost == "cost" else (c == 0).astype(np.float64)
-        parts = [perm[q * n // QUEUES:(q + 1) * n // QUEUES][np.argsort(key[q * n // QUEUES:(q + 1) * n // QUEUES],
-                                                                        kind="stable")] for q in range(QUEUES)]
+        parts = [perm[q * n // QUEUES:(q + 1) * n // QUEUES][np.argsort(-c[q * n // QUEUES:(q + 1) * n // QUEUES],
+                                                                      kind="stable")] for q in range(QUEUES)]
         perm = np.concatenate(parts)
     return perm
 
