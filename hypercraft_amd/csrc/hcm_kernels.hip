@@ -829,9 +829,13 @@ __device__ __forceinline__ void greedy_count(Smem<NT> &s, int t, bool active, ui
 		ev[k] = active ? evs[k] : 0u;
 	}
 	n_o = 0, n_t = 0;
+	// a row's masks are read one step ahead, so that their LDS latency overlaps the previous row's work
+	uint32_t ne_n = active ? s.ne[t][0] : 0u, eh_n = active ? s.eh[t][0] : 0u, tp_n = active ? s.tp[t][0] : 0u;
 #pragma unroll 1
 	for (int u = 0; u < 16; ++u) {
-		const uint32_t ne = active ? s.ne[t][u] : 0u, eh = active ? s.eh[t][u] : 0u, tp = active ? s.tp[t][u] : 0u;
+		const uint32_t ne = ne_n, eh = eh_n, tp = tp_n;
+		if (u < 15 && active) // row u + 1 is not written before this step writes row u
+			ne_n = s.ne[t][u + 1], eh_n = s.eh[t][u + 1], tp_n = s.tp[t][u + 1];
 		const uint32_t S = ne & ~(cov[0] & 0xffffu); // uncovered visible cells of row u
 		const uint32_t link = S & ((S & eh) << 1);   // v continues the run of v - 1
 		const uint32_t starts = S & ~link;
