@@ -916,18 +916,23 @@ __device__ __forceinline__ void quad_slot(uint32_t pass, uint32_t i, uint32_t n0
 
 // A written quad's box into its mesh's AABB (per lane for the first mesh of a pass, LDS atomics for the
 // others) and, when it starts a continuation mesh, a split event at its emission position epos.
+// A quad's box into the lane's box of the first mesh of its pass (static register indices only).
+__device__ __forceinline__ void box_add(uint32_t box[2][6], uint32_t pass, const uint32_t lo[3], const uint32_t hi[3]) {
+#pragma unroll
+	for (int a = 0; a < 3; ++a) {
+		box[0][a] = pass ? box[0][a] : min(box[0][a], lo[a]);
+		box[0][3 + a] = pass ? box[0][3 + a] : max(box[0][3 + a], hi[a]);
+		box[1][a] = pass ? min(box[1][a], lo[a]) : box[1][a];
+		box[1][3 + a] = pass ? max(box[1][3 + a], hi[a]) : box[1][3 + a];
+	}
+}
+
 template <int NT>
 __device__ __forceinline__ void quad_done(Smem<NT> &s, uint32_t pass, uint32_t mk, uint32_t mj, uint32_t epos,
                                           const uint32_t lo[3], const uint32_t hi[3], uint32_t box[2][6]) {
-	if (mk == 0) { // static register indices only
-#pragma unroll
-		for (int a = 0; a < 3; ++a) {
-			box[0][a] = pass ? box[0][a] : min(box[0][a], lo[a]);
-			box[0][3 + a] = pass ? box[0][3 + a] : max(box[0][3 + a], hi[a]);
-			box[1][a] = pass ? min(box[1][a], lo[a]) : box[1][a];
-			box[1][3 + a] = pass ? max(box[1][3 + a], hi[a]) : box[1][3 + a];
-		}
-	} else
+	if (mk == 0)
+		box_add(box, pass, lo, hi);
+	else
 		aabb_merge(s, pass, mk, lo, hi);
 	if (mj == 0 && mk > 0) { // a split happened right before this quad (:397-403)
 		const uint32_t ev = atomicAdd(&s.n_events, 1u);
@@ -945,8 +950,9 @@ __device__ __forceinline__ void quad_done(Smem<NT> &s, uint32_t pass, uint32_t m
 // quad from (u, v0) spans S's run from v0 to the next start along V, and covers rows u .. u + h - 1 of its
 // columns, so below it column v0 is in C exactly for rows u + 1 .. u + h - 1: the cell under its last row
 // is either not covered or starts a quad. Its index in its pass follows the pass's starts in the task's
-// earlier rows and columns (the reference's order: rows, then columns).
-template <int NT>
+// earlier rows and columns (the reference's order: rows, then columns). kSplit: some pass of the chunk
+// has more than kSplitQuads quads; without, every quad is in the first mesh of its pass.
+template <bool kSplit, int NT>
 __device__ __forceinline__ void emit_face(const MeshParams &p, Smem<NT> &s, uint32_t desc, uint2 word, uint32_t n0,
                                           uint32_t pool_base, uint32_t box[2][6]) {
 	const int t = desc & 127, u = (desc >> 7) & 15, v = (desc >> 11) & 15;
@@ -956,10 +962,13 @@ __device__ __forceinline__ void emit_face(const MeshParams &p, Smem<NT> &s, uint
 	const uint32_t tr = (st >> v) & 1u, below = (1u << v) - 1u;
 	const uint32_t link = uint32_t(s.ev[t][u]) & ~starts;
 	const int w = __ffs(~(link >> (v + 1)));
-	int h = 1;
+	int h = 1; // most quads are one row high: the row below is tested before the loop
+	if (u < 15 && ((uint32_t(s.ne[t][u + 1] & ~s.ev[t][u + 1]) >> v) & 1u)) {
+		h = 2;
 #pragma unroll 1
-	for (int r = u + 1; r < 16 && ((uint32_t(s.ne[t][r] & ~s.ev[t][r]) >> v) & 1u); ++r)
-		++h;
+		for (int r = u + 2; r < 16 && ((uint32_t(s.ne[t][r] & ~s.ev[t][r]) >> v) & 1u); ++r)
+			++h;
+	}
 	const uint64_t *m64 = reinterpret_cast<const uint64_t *>(tr ? s.tp[t] : s.eh[t]);
 	uint32_t ord = __popc((tr ? st : so) & below);
 #pragma unroll
@@ -971,6 +980,11 @@ __device__ __forceinline__ void emit_face(const MeshParams &p, Smem<NT> &s, uint
 	}
 	const uint32_t i = (tr ? s.qpre_t[t] : s.qpre_o[t]) + ord;
 	uint32_t slot, vbase, mk, mj, lo[3], hi[3];
+	if (!kSplit) {
+		emit_quad(p, s, t, u, v, w, h, word, pool_base + (tr ? n0 : 0u) + i, 4u * i, lo, hi);
+		box_add(box, tr, lo, hi);
+		return;
+	}
 	quad_slot(tr, i, n0, pool_base, slot, vbase, mk, mj);
 	emit_quad(p, s, t, u, v, w, h, word, slot, vbase, lo, hi);
 	uint32_t epos = 0;
@@ -985,6 +999,56 @@ __device__ __forceinline__ void emit_face(const MeshParams &p, Smem<NT> &s, uint
 		}
 	}
 	quad_done(s, tr, mk, mj, epos, lo, hi, box);
+}
+
+// The regular quads of a chunk, one lane per visible face (a quad per face that starts one): first the
+// last group of tasks, whose face words are still in key (keep: the lane's face positions), then the
+// earlier groups with their words computed again.
+template <bool kSplit, int NT>
+__device__ __forceinline__ void emit_regular(const MeshParams &p, Smem<NT> &s, uint32_t *keep, int t0_last, uint32_t n0,
+                                             uint32_t pool_base, uint32_t box[2][6]) {
+	const int tid = threadIdx.x;
+	constexpr int kMine = (kListCap + NT - 1) / NT;
+	{
+		const uint32_t g0 = s.fpre[t0_last], n = s.fpre[kTasks] - g0;
+#pragma unroll 2
+		for (int k = 0; k < kMine; ++k) {
+			const uint32_t i = tid + k * NT;
+			if (i < n)
+				emit_face<kSplit>(p, s, keep[0], s.key[i], n0, pool_base, box);
+#pragma unroll
+			for (int j = 0; j + 1 < kMine; ++j)
+				keep[j] = keep[j + 1];
+		}
+	}
+#pragma unroll 1
+	for (int t0 = 0; t0 < t0_last;) {
+		__syncthreads(); // key is read by the previous group's lanes
+		const int t1 = group_end(s, t0);
+		const uint32_t g0 = s.fpre[t0], n = s.fpre[t1] - g0;
+#pragma unroll 2
+		for (int k = 0; k < kMine; ++k) {
+			const uint32_t i = tid + k * NT;
+			uint32_t d = 0;
+			if (i < n)
+				s.key[i] = face_word(s, t0, t1, g0 + i, d);
+#pragma unroll
+			for (int j = 0; j + 1 < kMine; ++j)
+				keep[j] = keep[j + 1];
+			keep[kMine - 1] = d;
+		}
+		__syncthreads();
+#pragma unroll 2
+		for (int k = 0; k < kMine; ++k) {
+			const uint32_t i = tid + k * NT;
+			if (i < n)
+				emit_face<kSplit>(p, s, keep[0], s.key[i], n0, pool_base, box);
+#pragma unroll
+			for (int j = 0; j + 1 < kMine; ++j)
+				keep[j] = keep[j + 1];
+		}
+		t0 = t1;
+	}
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -1045,10 +1109,11 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 		if (row < NT)
 			wave_ref = v0;
 		const uint32_t ref2 = wave_ref | (wave_ref << 16);
-		bool nonuniform = bl != wave_ref || br != wave_ref;
+		uint32_t diff = (bl ^ wave_ref) | (br ^ wave_ref);
 #pragma unroll
 		for (int k = 0; k < 8; ++k)
-			nonuniform |= bw[k] != ref2;
+			diff |= bw[k] ^ ref2;
+		const bool nonuniform = diff != 0;
 		uint32_t tm[18];
 		tm[0] = s.tmpl_id[bl & 0xffu];
 		tm[17] = s.tmpl_id[br & 0xffu];
@@ -1244,17 +1309,26 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 			c_o += c.count - nt;
 		}
 	}
-	// a task with no equal neighbouring faces (no link bits) makes every visible face a 1 x 1 quad: its
-	// results follow from its masks directly. The others are listed and merged by one lane each, in as few
-	// wavefronts as they fill. The list passes through qpre_t and the counts through qpre_o (o | t << 16)
-	// before either holds its own values.
+	// A task whose equal neighbouring faces are all side by side (no equal-to-below bit) has one quad per
+	// run of equal faces along V: a visible face starts one unless it continues its left neighbour, and no
+	// cell is covered from above. One whose equal faces are all above each other (no equal-to-right bit)
+	// has one quad per run along U: a face starts one unless it continues the face above, and the cells
+	// below a start are covered by its quad. With neither, every visible face is a 1 x 1 quad. These follow
+	// from the masks directly (rows are 16-bit fields of the 64-bit words, row u + 4 q at bit 16 u, and a
+	// row's equal-to-right bit 15 is never set). Tasks with both are listed and merged by one lane each, in
+	// as few wavefronts as they fill. The list passes through qpre_t and the counts through qpre_o
+	// (o | t << 16) before either holds its own values.
 	if (tid < kTasks) { // the link masks are complete: the group loop ends with a barrier
 		const int t = tid;
-		uint64_t any = 0;
+		const uint64_t *eh64 = reinterpret_cast<const uint64_t *>(s.eh[t]);
+		const uint64_t *ev64 = reinterpret_cast<const uint64_t *>(s.ev[t]);
+		uint64_t eh[4], ev[4], anyh = 0, anyv = 0;
 #pragma unroll
-		for (int q = 0; q < 4; ++q)
-			any |= reinterpret_cast<const uint64_t *>(s.eh[t])[q] | reinterpret_cast<const uint64_t *>(s.ev[t])[q];
-		if (any) {
+		for (int q = 0; q < 4; ++q) {
+			eh[q] = eh64[q], ev[q] = ev64[q];
+			anyh |= eh[q], anyv |= ev[q];
+		}
+		if (anyh && anyv) {
 			s.qpre_t[atomicAdd(&s.n_link, 1u)] = uint32_t(t);
 		} else {
 			uint32_t no = 0, nt = 0;
@@ -1262,11 +1336,15 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 			for (int q = 0; q < 4; ++q) {
 				uint64_t *tp64 = reinterpret_cast<uint64_t *>(s.tp[t]) + q;
 				const uint64_t ne = reinterpret_cast<const uint64_t *>(s.ne[t])[q], tp = *tp64 & ne;
-				reinterpret_cast<uint64_t *>(s.ev[t])[q] = ne;       // uncovered visible cells
-				reinterpret_cast<uint64_t *>(s.eh[t])[q] = ne & ~tp; // opaque quads start here
-				*tp64 = tp;                                          // transparent quads start here
-				no += __popcll(ne & ~tp);
-				nt += __popcll(tp);
+				// equal-to-below bits of the row above each row: the word shifted up one row, with the last
+				// row of the previous word
+				const uint64_t above = (ev[q] << 16) | (q > 0 ? ev[q - 1] >> 48 : 0ull);
+				const uint64_t st = ne & ~(eh[q] << 1) & ~above;     // quad starts
+				reinterpret_cast<uint64_t *>(s.ev[t])[q] = anyv ? st : ne; // uncovered visible cells
+				reinterpret_cast<uint64_t *>(s.eh[t])[q] = st & ~tp;     // opaque quads start here
+				*tp64 = st & tp;                                         // transparent quads start here
+				no += __popcll(st & ~tp);
+				nt += __popcll(st & tp);
 			}
 			s.qpre_o[t] = no | nt << 16;
 		}
@@ -1329,48 +1407,11 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 #pragma unroll
 	for (int a = 0; a < 6; ++a)
 		box[0][a] = box[1][a] = a < 3 ? ~0u : 0u;
-	// regular quads, one lane per visible face (a quad per face that starts one): first the last group,
-	// whose face words are still in key, then the earlier groups with their words computed again
-	{
-		const uint32_t g0 = s.fpre[t0_last], n = s.fpre[kTasks] - g0;
-#pragma unroll 2
-		for (int k = 0; k < kMine; ++k) {
-			const uint32_t i = tid + k * NT;
-			if (i < n)
-				emit_face(p, s, keep[0], s.key[i], n0, pool_base, box);
-#pragma unroll
-			for (int j = 0; j + 1 < kMine; ++j)
-				keep[j] = keep[j + 1];
-		}
-	}
-#pragma unroll 1
-	for (int t0 = 0; t0 < t0_last;) {
-		__syncthreads(); // key is read by the previous group's lanes
-		const int t1 = group_end(s, t0);
-		const uint32_t g0 = s.fpre[t0], n = s.fpre[t1] - g0;
-#pragma unroll 2
-		for (int k = 0; k < kMine; ++k) {
-			const uint32_t i = tid + k * NT;
-			uint32_t d = 0;
-			if (i < n)
-				s.key[i] = face_word(s, t0, t1, g0 + i, d);
-#pragma unroll
-			for (int j = 0; j + 1 < kMine; ++j)
-				keep[j] = keep[j + 1];
-			keep[kMine - 1] = d;
-		}
-		__syncthreads();
-#pragma unroll 2
-		for (int k = 0; k < kMine; ++k) {
-			const uint32_t i = tid + k * NT;
-			if (i < n)
-				emit_face(p, s, keep[0], s.key[i], n0, pool_base, box);
-#pragma unroll
-			for (int j = 0; j + 1 < kMine; ++j)
-				keep[j] = keep[j + 1];
-		}
-		t0 = t1;
-	}
+	// regular quads, one lane per visible face (a quad per face that starts one)
+	if (n0 > kSplitQuads || n1 > kSplitQuads)
+		emit_regular<true>(p, s, keep, t0_last, n0, pool_base, box);
+	else
+		emit_regular<false>(p, s, keep, t0_last, n0, pool_base, box);
 	// custom quads, after the regular ones (their positions in the output are fixed by the prefix sums): in
 	// windows of the face-word stage, the threads list their cells' quads (cells in YZX order, faces in
 	// mesh order, each thread its range of cells), then one lane per listed quad writes it
