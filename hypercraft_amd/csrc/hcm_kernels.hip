@@ -102,6 +102,19 @@ __device__ __forceinline__ int slot27(int x, int y, int z) {
 	int cx = x < 0 ? -1 : (x >= 16 ? 1 : 0), cy = y < 0 ? -1 : (y >= 16 ? 1 : 0), cz = z < 0 ? -1 : (z >= 16 ? 1 : 0);
 	return ((cx + 2) % 3) * 9 + ((cy + 2) % 3) * 3 + (cz + 2) % 3;
 }
+// slot27's y and z terms for the halo row (hy, hz) = (y + 1, z + 1): a halo coordinate h contributes 1 at
+// h = 0 (-1), 2 at h = 1 .. 16 (0) and 0 at h = 17 (16), two bits per h in kSlotTerm.
+constexpr uint64_t slot_term_table() {
+	uint64_t c = 0;
+	for (int h = 0; h < 18; ++h)
+		c |= uint64_t(h == 0 ? 1 : (h == 17 ? 0 : 2)) << (2 * h);
+	return c;
+}
+constexpr uint64_t kSlotTerm = slot_term_table();
+static_assert(kSlotTerm == 0x2aaaaaaa9ull, "slot27 halo-coordinate terms");
+__device__ __forceinline__ int slot27_yz(int hy, int hz) {
+	return 3 * int((kSlotTerm >> (2 * hy)) & 3u) + int((kSlotTerm >> (2 * hz)) & 3u);
+}
 __device__ __forceinline__ uint32_t local16(int x, int y, int z) {
 	return uint32_t(x & 15) + 16u * (uint32_t(z & 15) + 16u * uint32_t(y & 15));
 }
@@ -657,19 +670,19 @@ template <int A, int NT> __device__ __forceinline__ void visibility_column(Smem<
 	for (int a = 0; a < kHalo; ++a)
 		c[a] = (cw[a * sa] >> shp) & 15u;
 	const int q = (col >> 6) & 3; // this wave's group of four mask rows
-	uint64_t tt = 0;              // lanes with transparent against transparent, over all slices
+	uint32_t tt_bits = 0;         // 1 << i over all slices: bit kShowTT marks transparent against transparent
 #pragma unroll
 	for (int a = 0; a <= 16; ++a) {
 		const uint32_t i = (c[a] & 3u) | (c[a + 1] & 12u);
 		const uint64_t bp = a >= 1 ? ballot64((kShowPlus >> i) & 1u) : 0ull;
 		const uint64_t bm = a <= 15 ? ballot64((kShowMinus >> i) & 1u) : 0ull;
-		tt |= ballot64(i == kShowTT);
+		tt_bits |= 1u << i;
 		if (lane == 0) {
 			reinterpret_cast<uint64_t *>(s.ne[34 * A + 2 * a])[q] = bp;
 			reinterpret_cast<uint64_t *>(s.ne[34 * A + 2 * a + 1])[q] = bm;
 		}
 	}
-	if (tt) { // such faces show when the two texture codes differ (both directions alike)
+	if (ballot64((tt_bits >> kShowTT) & 1u)) { // such faces show when the two texture codes differ (both directions alike)
 #pragma unroll 1
 		for (int a = 0; a <= 16; ++a) {
 			const uint32_t wa = cw[a * sa], wb = cw[(a + 1) * sa];
@@ -695,39 +708,31 @@ __device__ __forceinline__ int face_owner(int t, int u, int v, int &face) {
 }
 
 // Task, mask row and column of the visible face with dense index gi (task-major, row-major, as the
-// reference scans them), searched among the tasks [t0, t1).
+// reference scans them), searched among the tasks [t0, t1), without branches: the task by seven halving
+// steps over the prefix (the last t with fpre[t] <= gi; fpre[t1] > gi and a group spans under 128 tasks),
+// then the task's four 64-bit mask words and the word's four 16-bit rows by comparing the remaining index
+// with their running counts, then the column as the rem-th set bit of the row.
 template <int NT> __device__ __forceinline__ void face_pos(const Smem<NT> &s, int t0, int t1, uint32_t gi, int &t, int &u, int &v) {
-	int lo = t0, hi = t1 - 1; // the task holding gi: fpre[t] <= gi < fpre[t + 1]
-	while (lo < hi) {
-		const int mid = (lo + hi + 1) >> 1;
-		if (s.fpre[mid] <= gi)
-			lo = mid;
-		else
-			hi = mid - 1;
+	int lo = t0;
+#pragma unroll
+	for (int step = 64; step >= 1; step >>= 1) {
+		const int m = min(lo + step, t1);
+		lo = s.fpre[m] <= gi ? m : lo;
 	}
 	t = lo;
 	uint32_t rem = gi - s.fpre[t];
 	const uint64_t *rows = reinterpret_cast<const uint64_t *>(s.ne[t]);
-	u = 0;
-	uint32_t bits = 0;
-#pragma unroll 1
-	for (int q = 0; q < 4; ++q) { // the row holding it
-		const uint64_t w = rows[q];
-		const uint32_t c = __popcll(w);
-		if (rem >= c) {
-			rem -= c;
-			continue;
-		}
-#pragma unroll
-		for (int r = 0; r < 4; ++r) {
-			const uint32_t row = uint32_t(w >> (16 * r)) & 0xffffu, cr = __popc(row);
-			if (bits == 0 && rem < cr)
-				u = 4 * q + r, bits = row;
-			else if (bits == 0)
-				rem -= cr;
-		}
-		break;
-	}
+	const uint64_t w0 = rows[0], w1 = rows[1], w2 = rows[2], w3 = rows[3];
+	const uint32_t p1 = __popcll(w0), p2 = p1 + __popcll(w1), p3 = p2 + __popcll(w2);
+	const bool q1 = rem >= p1, q2 = rem >= p2, q3 = rem >= p3;
+	const uint64_t w = q3 ? w3 : (q2 ? w2 : (q1 ? w1 : w0));
+	rem -= q3 ? p3 : (q2 ? p2 : (q1 ? p1 : 0u));
+	const uint32_t lo32 = uint32_t(w), hi32 = uint32_t(w >> 32);
+	const uint32_t r1 = __popc(lo32 & 0xffffu), r2 = __popc(lo32), r3 = r2 + __popc(hi32 & 0xffffu);
+	const bool s1 = rem >= r1, s2 = rem >= r2, s3 = rem >= r3;
+	uint32_t bits = s3 ? hi32 >> 16 : (s2 ? hi32 & 0xffffu : (s1 ? lo32 >> 16 : lo32 & 0xffffu));
+	rem -= s3 ? r3 : (s2 ? r2 : (s1 ? r1 : 0u));
+	u = 4 * (int(q1) + int(q2) + int(q3)) + int(s1) + int(s2) + int(s3);
 	// the column: the rem-th set bit of the row
 	v = 0;
 	const uint32_t c8 = __popc(bits & 0xffu);
@@ -962,8 +967,12 @@ __device__ __forceinline__ void emit_face(const MeshParams &p, Smem<NT> &s, uint
 	const uint32_t tr = (st >> v) & 1u, below = (1u << v) - 1u;
 	const uint32_t link = uint32_t(s.ev[t][u]) & ~starts;
 	const int w = __ffs(~(link >> (v + 1)));
-	int h = 1; // most quads are one row high: the row below is tested before the loop
-	if (u < 15 && ((uint32_t(s.ne[t][u + 1] & ~s.ev[t][u + 1]) >> v) & 1u)) {
+	// most quads are one row high: the covered cells of the row below are tested before the loop (row 15
+	// has none; its index is clamped so that the load needs no branch)
+	const int ub = min(u + 1, 15);
+	const uint32_t cov_below = uint32_t(s.ne[t][ub] & ~s.ev[t][ub]) & (0u - uint32_t(u < 15));
+	int h = 1;
+	if ((cov_below >> v) & 1u) {
 		h = 2;
 #pragma unroll 1
 		for (int r = u + 2; r < 16 && ((uint32_t(s.ne[t][r] & ~s.ev[t][r]) >> v) & 1u); ++r)
@@ -972,11 +981,10 @@ __device__ __forceinline__ void emit_face(const MeshParams &p, Smem<NT> &s, uint
 	const uint64_t *m64 = reinterpret_cast<const uint64_t *>(tr ? s.tp[t] : s.eh[t]);
 	uint32_t ord = __popc((tr ? st : so) & below);
 #pragma unroll
-	for (int q = 0; q < 4; ++q) { // the pass's starts in rows 0 .. u - 1
+	for (int q = 0; q < 4; ++q) { // the pass's starts in rows 0 .. u - 1 (all four words read, no branch)
 		const int nb = 16 * (u - 4 * q);
 		const uint64_t m = nb >= 64 ? ~0ull : (nb > 0 ? (1ull << nb) - 1ull : 0ull);
-		if (nb > 0)
-			ord += __popcll(m64[q] & m);
+		ord += __popcll(m64[q] & m);
 	}
 	const uint32_t i = (tr ? s.qpre_t[t] : s.qpre_o[t]) + ord;
 	uint32_t slot, vbase, mk, mj, lo[3], hi[3];
@@ -1080,9 +1088,10 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 	for (int row = tid; row < kHaloRows; row += NT) {
 		const int hz = row % kHalo, hy = row / kHalo, y = hy - 1, z = hz - 1;
 		const size_t lrow = 16u * (uint32_t(z & 15) + 16u * uint32_t(y & 15));
-		const size_t cm = (size_t)s.slots[slot27(0, y, z)] * HCM_CHUNK_VOXELS + lrow;
-		const size_t cl = (size_t)s.slots[slot27(-1, y, z)] * HCM_CHUNK_VOXELS + lrow + 15;
-		const size_t cr = (size_t)s.slots[slot27(16, y, z)] * HCM_CHUNK_VOXELS + lrow;
+		const int syz = slot27_yz(hy, hz); // slot27(x, y, z) = 9 * (x term) + syz: 18 at x = 0, 9 at -1, 0 at 16
+		const size_t cm = (size_t)s.slots[18 + syz] * HCM_CHUNK_VOXELS + lrow;
+		const size_t cl = (size_t)s.slots[9 + syz] * HCM_CHUNK_VOXELS + lrow + 15;
+		const size_t cr = (size_t)s.slots[syz] * HCM_CHUNK_VOXELS + lrow;
 		const uint4 b0 = *reinterpret_cast<const uint4 *>(p.blocks + cm);
 		const uint4 b1 = *reinterpret_cast<const uint4 *>(p.blocks + cm + 8);
 		const uint32_t bw[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
