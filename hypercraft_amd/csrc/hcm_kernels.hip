@@ -1180,10 +1180,17 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 			my_flags |= kFlagCustom;
 		}
 	}
-	if (my_flags)
-		atomicOr(&s.flags, my_flags);
-	if (lane == 0)
+	// the wavefront's flags, bit by bit (kFlagNonUniform .. kFlagInnerFaces), then one lane's LDS atomic:
+	// an atomic from every lane with a different value is expanded into a scalar loop over the lanes
+	uint32_t wave_flags = 0;
+#pragma unroll
+	for (int b = 0; b < 5; ++b)
+		wave_flags |= ballot64((my_flags >> b) & 1u) ? 1u << b : 0u;
+	if (lane == 0) {
+		if (wave_flags)
+			atomicOr(&s.flags, wave_flags);
 		s.wave_ref[wave] = wave_ref; // 0xffffffff: the wavefront staged no row
+	}
 	__syncthreads();
 	uint32_t flags = s.flags;
 #pragma unroll
