@@ -1016,16 +1016,18 @@ template <bool kSplit, int NT>
 __device__ __forceinline__ void emit_regular(const MeshParams &p, Smem<NT> &s, uint32_t *keep, int t0_last, uint32_t n0,
                                              uint32_t pool_base, uint32_t box[2][6]) {
 	const int tid = threadIdx.x;
-	constexpr int kMine = (kListCap + NT - 1) / NT;
+	constexpr int kMine = (kListCap + NT - 1) / NT, kPairs = (kMine + 1) / 2;
 	{
 		const uint32_t g0 = s.fpre[t0_last], n = s.fpre[kTasks] - g0;
-#pragma unroll 2
-		for (int k = 0; k < kMine; ++k) {
-			const uint32_t i = tid + k * NT;
-			if (i < n)
-				emit_face<kSplit>(p, s, keep[0], s.key[i], n0, pool_base, box);
+#pragma unroll 1
+		for (int k = 0; k < kPairs; ++k) {
+			const uint32_t i0 = tid + 2 * k * NT, i1 = i0 + NT;
+			if (i0 < n)
+				emit_face<kSplit>(p, s, keep[0] & 0xffffu, s.key[i0], n0, pool_base, box);
+			if (i1 < n)
+				emit_face<kSplit>(p, s, keep[0] >> 16, s.key[i1], n0, pool_base, box);
 #pragma unroll
-			for (int j = 0; j + 1 < kMine; ++j)
+			for (int j = 0; j + 1 < kPairs; ++j)
 				keep[j] = keep[j + 1];
 		}
 	}
@@ -1034,25 +1036,29 @@ __device__ __forceinline__ void emit_regular(const MeshParams &p, Smem<NT> &s, u
 		__syncthreads(); // key is read by the previous group's lanes
 		const int t1 = group_end(s, t0);
 		const uint32_t g0 = s.fpre[t0], n = s.fpre[t1] - g0;
-#pragma unroll 2
-		for (int k = 0; k < kMine; ++k) {
-			const uint32_t i = tid + k * NT;
-			uint32_t d = 0;
-			if (i < n)
-				s.key[i] = face_word(s, t0, t1, g0 + i, d);
+#pragma unroll 1
+		for (int k = 0; k < kPairs; ++k) {
+			const uint32_t i0 = tid + 2 * k * NT, i1 = i0 + NT;
+			uint32_t d0 = 0, d1 = 0;
+			if (i0 < n)
+				s.key[i0] = face_word(s, t0, t1, g0 + i0, d0);
+			if (i1 < n)
+				s.key[i1] = face_word(s, t0, t1, g0 + i1, d1);
 #pragma unroll
-			for (int j = 0; j + 1 < kMine; ++j)
+			for (int j = 0; j + 1 < kPairs; ++j)
 				keep[j] = keep[j + 1];
-			keep[kMine - 1] = d;
+			keep[kPairs - 1] = d0 | d1 << 16;
 		}
 		__syncthreads();
-#pragma unroll 2
-		for (int k = 0; k < kMine; ++k) {
-			const uint32_t i = tid + k * NT;
-			if (i < n)
-				emit_face<kSplit>(p, s, keep[0], s.key[i], n0, pool_base, box);
+#pragma unroll 1
+		for (int k = 0; k < kPairs; ++k) {
+			const uint32_t i0 = tid + 2 * k * NT, i1 = i0 + NT;
+			if (i0 < n)
+				emit_face<kSplit>(p, s, keep[0] & 0xffffu, s.key[i0], n0, pool_base, box);
+			if (i1 < n)
+				emit_face<kSplit>(p, s, keep[0] >> 16, s.key[i1], n0, pool_base, box);
 #pragma unroll
-			for (int j = 0; j + 1 < kMine; ++j)
+			for (int j = 0; j + 1 < kPairs; ++j)
 				keep[j] = keep[j + 1];
 		}
 		t0 = t1;
@@ -1266,10 +1272,13 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 			p.results[job] = hcm_job_result{HCM_OK, 0u, 0u, 0u};
 		return;
 	}
-	// face words per group of tasks, then the group's link masks; the last group's words stay in key
-	constexpr int kMine = (kListCap + NT - 1) / NT;
+	// face words per group of tasks, then the group's link masks; the last group's words stay in key. A
+	// lane keeps the positions of its faces (at most kListCap / NT of them, 15 bits each, two to a
+	// register) for the links and the emission, in a register queue shifted once per two faces (static
+	// register indices)
+	constexpr int kMine = (kListCap + NT - 1) / NT, kPairs = (kMine + 1) / 2;
 	int t0_last = 0;
-	uint32_t keep[kMine]; // the positions of this lane's faces of the last group, for the emission
+	uint32_t keep[kPairs]; // the positions of this lane's faces of the last group, for the emission
 #pragma unroll 1
 	for (int t0 = 0; t0 < kTasks;) {
 		const int t1 = group_end(s, t0);
@@ -1280,31 +1289,33 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 			reinterpret_cast<uint32_t *>(s.ev[t])[q] = 0;
 			reinterpret_cast<uint32_t *>(s.tp[t])[q] = 0;
 		}
-		// a lane keeps the positions of its faces (at most kListCap / NT of them) for the links, in a
-		// register queue shifted once per step (loops unrolled by two, static register indices)
-		uint32_t mine[kMine] = {};
-#pragma unroll 2
-		for (int k = 0; k < kMine; ++k) {
-			const uint32_t i = tid + k * NT;
-			uint32_t d = 0;
-			if (i < n)
-				s.key[i] = face_word(s, t0, t1, g0 + i, d);
+		uint32_t mine[kPairs] = {};
+#pragma unroll 1
+		for (int k = 0; k < kPairs; ++k) {
+			const uint32_t i0 = tid + 2 * k * NT, i1 = i0 + NT;
+			uint32_t d0 = 0, d1 = 0;
+			if (i0 < n)
+				s.key[i0] = face_word(s, t0, t1, g0 + i0, d0);
+			if (i1 < n)
+				s.key[i1] = face_word(s, t0, t1, g0 + i1, d1);
 #pragma unroll
-			for (int j = 0; j + 1 < kMine; ++j)
+			for (int j = 0; j + 1 < kPairs; ++j)
 				mine[j] = mine[j + 1];
-			mine[kMine - 1] = d;
+			mine[kPairs - 1] = d0 | d1 << 16;
 		}
 #pragma unroll
-		for (int j = 0; j < kMine; ++j)
+		for (int j = 0; j < kPairs; ++j)
 			keep[j] = mine[j];
 		__syncthreads();
-#pragma unroll 2
-		for (int k = 0; k < kMine; ++k) {
-			const uint32_t i = tid + k * NT;
-			if (i < n)
-				face_links(s, i, mine[0]);
+#pragma unroll 1
+		for (int k = 0; k < kPairs; ++k) {
+			const uint32_t i0 = tid + 2 * k * NT, i1 = i0 + NT;
+			if (i0 < n)
+				face_links(s, i0, mine[0] & 0xffffu);
+			if (i1 < n)
+				face_links(s, i1, mine[0] >> 16);
 #pragma unroll
-			for (int j = 0; j + 1 < kMine; ++j)
+			for (int j = 0; j + 1 < kPairs; ++j)
 				mine[j] = mine[j + 1];
 		}
 		__syncthreads();
