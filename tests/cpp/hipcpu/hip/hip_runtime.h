@@ -112,6 +112,8 @@ inline int __popc(uint32_t v) { return __builtin_popcount(v); }
 inline int __popcll(uint64_t v) { return __builtin_popcountll(v); }
 inline int __ffs(uint32_t v) { return __builtin_ffs((int)v); }
 inline int __ffs(int v) { return __builtin_ffs(v); }
+inline int __clz(int v) { return v ? __builtin_clz((unsigned)v) : 32; }
+inline int __clzll(long long v) { return v ? __builtin_clzll((unsigned long long)v) : 64; }
 inline int __ffsll(unsigned long long v) { return __builtin_ffsll((long long)v); }
 inline int __ffsll(long long v) { return __builtin_ffsll(v); }
 inline int __ffsll(unsigned long v) { return __builtin_ffsll((long long)v); }
