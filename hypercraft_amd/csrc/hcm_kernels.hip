@@ -138,7 +138,7 @@ template <int NT> struct Smem {
 	uint32_t tmpl_id[256];     // voxel-word template by block id when the meta byte does not change it, else kTmplMeta
 	uint32_t aabb[2][kMaxMeshPerPass][6];
 	uint32_t events[16][2];
-	uint64_t scan2[NT / 64][2];
+	uint32_t scan_w[NT / 64][4]; // per wavefront: its totals of block_excl_sums
 	uint32_t slots[27];
 	uint32_t job, flags, n_events, n_link;
 	uint32_t wave_ref[NT / 64];
@@ -279,35 +279,48 @@ template <bool kMax> __device__ __forceinline__ uint32_t wave_fold(uint32_t v) {
 }
 
 // ------------------------------------------------------------------------------------------------
-// block-wide exclusive scan of 64-bit values (two 32-bit counters packed lo | hi << 32)
+// block-wide exclusive scan of three 32-bit counters (every thread reads s.scan_w after the barrier; the
+// next write to it is a later chunk's, after several more barriers)
 // ------------------------------------------------------------------------------------------------
-// Exclusive prefix sums of two values over the workgroup's threads in one pass (one barrier): returns
-// each thread's prefixes in a and b and the totals in ta and tb. Every thread reads s.scan2 after the
-// barrier; the next write to it is a later chunk's, after several more barriers.
+// Inclusive prefix sum over the wavefront by DPP: shifts by 1, 2, 4 and 8 lanes within each row of 16,
+// then the last lane of row 0 into row 1 (and of row 2 into row 3), then lane 31 into rows 2 and 3.
+__device__ __forceinline__ uint32_t wave_incl_sum(uint32_t v) {
+	v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, true); // row_shr:1
+	v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, true); // row_shr:2
+	v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, true); // row_shr:4
+	v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, true); // row_shr:8
+	v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, true); // row_bcast:15
+	v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, true); // row_bcast:31
+	return v;
+}
+
+// Exclusive prefix sums over the workgroup of three 32-bit values per thread, and their totals, with one
+// barrier.
 template <int NT>
-__device__ __forceinline__ void block_excl_scan2(Smem<NT> &s, uint64_t &a, uint64_t &b, uint64_t &ta, uint64_t &tb) {
+__device__ __forceinline__ void block_excl_sums(Smem<NT> &s, uint32_t v[3], uint32_t tot[3]) {
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-	uint64_t ia = a, ib = b;
+	uint32_t inc[3];
 #pragma unroll
-	for (int d = 1; d < 64; d <<= 1) {
-		const uint32_t al = __shfl_up((uint32_t)ia, d), ah = __shfl_up((uint32_t)(ia >> 32), d);
-		const uint32_t bl = __shfl_up((uint32_t)ib, d), bh = __shfl_up((uint32_t)(ib >> 32), d);
-		if (lane >= d) {
-			ia += (uint64_t)al | ((uint64_t)ah << 32);
-			ib += (uint64_t)bl | ((uint64_t)bh << 32);
-		}
+	for (int i = 0; i < 3; ++i)
+		inc[i] = wave_incl_sum(v[i]);
+	if (lane == 63) {
+#pragma unroll
+		for (int i = 0; i < 3; ++i)
+			s.scan_w[wave][i] = inc[i];
 	}
-	if (lane == 63)
-		s.scan2[wave][0] = ia, s.scan2[wave][1] = ib;
 	__syncthreads();
-	uint64_t ra = 0, rb = 0, sa = 0, sb = 0;
 #pragma unroll
-	for (int w = 0; w < NT / 64; ++w) {
-		const uint64_t xa = s.scan2[w][0], xb = s.scan2[w][1];
-		ra += w < wave ? xa : 0, rb += w < wave ? xb : 0;
-		sa += xa, sb += xb;
+	for (int i = 0; i < 3; ++i) {
+		uint32_t before = 0, all = 0;
+#pragma unroll
+		for (int w = 0; w < NT / 64; ++w) {
+			const uint32_t x = s.scan_w[w][i];
+			before += w < wave ? x : 0u;
+			all += x;
+		}
+		v[i] = before + inc[i] - v[i];
+		tot[i] = all;
 	}
-	a = ra + ia - a, b = rb + ib - b, ta = sa, tb = sb;
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -1395,11 +1408,12 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 	}
 	// 5. block-wide prefix sums and allocation: the chunk's output is [pool_base, +n0) opaque and
 	// [+n0, +n0+n1) transparent; custom quads come first in each pass, then the tasks' quads in task order
-	uint64_t cust_pre = (uint64_t)c_o | ((uint64_t)c_t << 32), task_pre = (uint64_t)q_o | ((uint64_t)q_t << 32);
-	uint64_t cust_total, task_total;
-	block_excl_scan2(s, cust_pre, task_pre, cust_total, task_total);
-	const uint32_t cust0 = (uint32_t)cust_total, cust1 = (uint32_t)(cust_total >> 32);
-	const uint32_t n0 = cust0 + (uint32_t)task_total, n1 = cust1 + (uint32_t)(task_total >> 32);
+	// (a chunk's regular quads per pass are at most its 24,576 faces, so both passes' counts share a word)
+	uint32_t scan[3] = {q_o | q_t << 16, c_o, c_t}, tot[3];
+	block_excl_sums(s, scan, tot);
+	const uint64_t cust_pre = (uint64_t)scan[1] | ((uint64_t)scan[2] << 32);
+	const uint32_t cust0 = tot[1], cust1 = tot[2];
+	const uint32_t n0 = cust0 + (tot[0] & 0xffffu), n1 = cust1 + (tot[0] >> 16);
 	const uint32_t total = n0 + n1;
 	if (total == 0) {
 		if (tid == 0)
@@ -1407,8 +1421,8 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 		return;
 	}
 	if (tid < kTasks) {
-		s.qpre_o[tid] = cust0 + (uint32_t)task_pre;
-		s.qpre_t[tid] = cust1 + (uint32_t)(task_pre >> 32);
+		s.qpre_o[tid] = cust0 + (scan[0] & 0xffffu);
+		s.qpre_t[tid] = cust1 + (scan[0] >> 16);
 	}
 	// allocation: the quad range and the mesh records, one atomic each, issued by two lanes of one
 	// instruction so that both are in flight together
@@ -1475,11 +1489,17 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 		}
 	}
 
-#pragma unroll
-	for (int pass = 0; pass < 2; ++pass)
+	// the passes' boxes over the wavefront (a pass without quads is skipped: its boxes are empty)
+	if (n0) {
 #pragma unroll
 		for (int a = 0; a < 6; ++a)
-			box[pass][a] = a < 3 ? wave_fold<false>(box[pass][a]) : wave_fold<true>(box[pass][a]);
+			box[0][a] = a < 3 ? wave_fold<false>(box[0][a]) : wave_fold<true>(box[0][a]);
+	}
+	if (n1) {
+#pragma unroll
+		for (int a = 0; a < 6; ++a)
+			box[1][a] = a < 3 ? wave_fold<false>(box[1][a]) : wave_fold<true>(box[1][a]);
+	}
 	if (lane == 0 && (n0 | n1)) {
 		if (n0)
 			aabb_merge(s, 0, 0, box[0], box[0] + 3);

@@ -41,6 +41,11 @@ static int dpp_src(uint32_t ctrl, int lane) {
 	switch (ctrl) {
 	case 0x140: return (row << 4) | (15 - in);                  // row_mirror
 	case 0x141: return (row << 4) | ((in & 8) | (7 - (in & 7))); // row_half_mirror
+	case 0x111: case 0x112: case 0x113: case 0x114: case 0x115: case 0x116: case 0x117: case 0x118:
+	case 0x119: case 0x11a: case 0x11b: case 0x11c: case 0x11d: case 0x11e: case 0x11f: { // row_shr:n
+		const int n = int(ctrl & 15u);
+		return in >= n ? lane - n : -1;
+	}
 	case 0x142: return row >= 1 ? ((row - 1) << 4) | 15 : -1;    // row_bcast:15
 	case 0x143: return row >= 2 ? 31 : -1;                       // row_bcast:31
 	default:
