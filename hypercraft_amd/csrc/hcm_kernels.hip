@@ -1254,30 +1254,25 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 	for (int col = tid; col < 256; col += NT)
 		visibility_column<2>(s, col);
 	__syncthreads();
-	if (wave == 0) { // exclusive prefix of visible faces over the tasks
-		uint32_t a = 0, b = 0;
+	if (wave == 0) { // exclusive prefix of visible faces over the tasks: lane l counts tasks l and l + 64,
+	                 // packed in one word (a chunk has at most 24,576 visible faces), and one DPP scan
+		uint32_t c = 0;
 		if (lane < kTasks) {
 			const uint64_t *r = reinterpret_cast<const uint64_t *>(s.ne[lane]);
-			a = __popcll(r[0]) + __popcll(r[1]) + __popcll(r[2]) + __popcll(r[3]);
+			c = __popcll(r[0]) + __popcll(r[1]) + __popcll(r[2]) + __popcll(r[3]);
 		}
 		if (lane + 64 < kTasks) {
 			const uint64_t *r = reinterpret_cast<const uint64_t *>(s.ne[lane + 64]);
-			b = __popcll(r[0]) + __popcll(r[1]) + __popcll(r[2]) + __popcll(r[3]);
+			c |= uint32_t(__popcll(r[0]) + __popcll(r[1]) + __popcll(r[2]) + __popcll(r[3])) << 16;
 		}
-		uint32_t ia = a, ib = b;
-#pragma unroll
-		for (int d = 1; d < 64; d <<= 1) {
-			const uint32_t xa = __shfl_up(ia, d), xb = __shfl_up(ib, d);
-			if (lane >= d)
-				ia += xa, ib += xb;
-		}
-		const uint32_t tot_a = __shfl(ia, 63);
+		const uint32_t inc = wave_incl_sum(c), ex = inc - c;
+		const uint32_t tot = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63), tot_a = tot & 0xffffu;
 		if (lane < kTasks)
-			s.fpre[lane] = ia - a;
+			s.fpre[lane] = ex & 0xffffu;
 		if (lane + 64 < kTasks)
-			s.fpre[lane + 64] = tot_a + ib - b;
+			s.fpre[lane + 64] = tot_a + (ex >> 16);
 		if (lane == 63)
-			s.fpre[kTasks] = tot_a + ib;
+			s.fpre[kTasks] = tot_a + (tot >> 16);
 	}
 	__syncthreads();
 	if (s.fpre[kTasks] == 0 && !(flags & kFlagCustom)) { // no visible face and no custom mesh: no quads
@@ -1586,7 +1581,7 @@ template <int NT> __global__ __launch_bounds__(NT, NT >= 512 ? 2 : 3) void mesh_
 				}
 				s.job = job;
 			}
-			job = (uint32_t)__shfl((int)job, 0);
+			job = (uint32_t)__builtin_amdgcn_readfirstlane((int)job);
 			if (job < p.n_jobs && lane < 27)
 				s.slots[lane] = p.nbr[27 * (size_t)job + lane];
 		}
