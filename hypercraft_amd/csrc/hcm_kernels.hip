@@ -1023,8 +1023,9 @@ __device__ __forceinline__ void emit_face(const MeshParams &p, Smem<NT> &s, uint
 }
 
 // The regular quads of a chunk, one lane per visible face (a quad per face that starts one): first the
-// last group of tasks, whose face words are still in key (keep: the lane's face positions), then the
-// earlier groups with their words computed again.
+// last group of tasks, whose face words are still in key (keep: the lane's face positions, two 15-bit
+// positions to a register, faces tid + 2 k NT and tid + (2 k + 1) NT in register k), then the earlier
+// groups with their words computed again.
 template <bool kSplit, int NT>
 __device__ __forceinline__ void emit_regular(const MeshParams &p, Smem<NT> &s, uint32_t *keep, int t0_last, uint32_t n0,
                                              uint32_t pool_base, uint32_t box[2][6]) {
