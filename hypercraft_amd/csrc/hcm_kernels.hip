@@ -783,20 +783,20 @@ __device__ __forceinline__ uint2 face_word(const Smem<NT> &s, int t0, int t1, ui
 // i + 1 and after the rest of row u and the start of row u + 1 in the dense order.
 template <int NT> __device__ __forceinline__ void face_links(Smem<NT> &s, uint32_t i, uint32_t desc) {
 	const int t = desc & 127, u = (desc >> 7) & 15, v = (desc >> 11) & 15;
-	const uint32_t ne_u = s.ne[t][u], ne_d = u < 15 ? s.ne[t][u + 1] : 0u;
+	// the neighbours' words are read whether or not the neighbours are visible (positions clamped into the
+	// stage) and compared as whole words, so that only the rare atomics sit behind branches
+	const uint32_t ne_u = s.ne[t][u], ne_d = uint32_t(s.ne[t][min(u + 1, 15)]) & (0u - uint32_t(u < 15));
 	const uint2 w = s.key[i];
-	uint32_t *row = reinterpret_cast<uint32_t *>(&s.eh[t][u & ~1]);
 	const uint32_t bit = 1u << (v + 16 * (u & 1));
-	if (v < 15 && ((ne_u >> (v + 1)) & 1u)) {
-		const uint2 r = s.key[i + 1];
-		if (r.x == w.x && r.y == w.y)
-			atomicOr(row, bit);
-	}
-	if ((ne_d >> v) & 1u) {
-		const uint2 d = s.key[i + 1 + __popc(ne_u >> (v + 1)) + __popc(ne_d & ((1u << v) - 1u))];
-		if (d.x == w.x && d.y == w.y)
-			atomicOr(reinterpret_cast<uint32_t *>(&s.ev[t][u & ~1]), bit);
-	}
+	const uint32_t ir = min(i + 1, uint32_t(kListCap - 1));
+	const uint32_t id = min(i + 1 + __popc(ne_u >> (v + 1)) + __popc(ne_d & ((1u << v) - 1u)), uint32_t(kListCap - 1));
+	const uint2 r = s.key[ir], d = s.key[id];
+	const bool eq_r = v < 15 && ((ne_u >> (v + 1)) & 1u) && ((r.x ^ w.x) | (r.y ^ w.y)) == 0;
+	const bool eq_d = ((ne_d >> v) & 1u) && ((d.x ^ w.x) | (d.y ^ w.y)) == 0;
+	if (eq_r)
+		atomicOr(reinterpret_cast<uint32_t *>(&s.eh[t][u & ~1]), bit);
+	if (eq_d)
+		atomicOr(reinterpret_cast<uint32_t *>(&s.ev[t][u & ~1]), bit);
 	if (w.y >> 31)
 		atomicOr(reinterpret_cast<uint32_t *>(&s.tp[t][u & ~1]), bit);
 }
@@ -991,13 +991,15 @@ __device__ __forceinline__ void emit_face(const MeshParams &p, Smem<NT> &s, uint
 		for (int r = u + 2; r < 16 && ((uint32_t(s.ne[t][r] & ~s.ev[t][r]) >> v) & 1u); ++r)
 			++h;
 	}
-	const uint64_t *m64 = reinterpret_cast<const uint64_t *>(tr ? s.tp[t] : s.eh[t]);
+	// the pass's starts in rows 0 .. u - 1: eight words of two rows each (all read, no branch); word k
+	// counts whole when 2 k + 2 <= u, its low row only when 2 k + 1 == u
+	const uint32_t *m32 = reinterpret_cast<const uint32_t *>(tr ? s.tp[t] : s.eh[t]);
 	uint32_t ord = __popc((tr ? st : so) & below);
 #pragma unroll
-	for (int q = 0; q < 4; ++q) { // the pass's starts in rows 0 .. u - 1 (all four words read, no branch)
-		const int nb = 16 * (u - 4 * q);
-		const uint64_t m = nb >= 64 ? ~0ull : (nb > 0 ? (1ull << nb) - 1ull : 0ull);
-		ord += __popcll(m64[q] & m);
+	for (int k = 0; k < 8; ++k) {
+		const int d = u - 2 * k; // rows of word k below row u: 2 when d >= 2, 1 when d == 1
+		const uint32_t wk = m32[k];
+		ord += __popc(d >= 2 ? wk : (d == 1 ? wk & 0xffffu : 0u));
 	}
 	const uint32_t i = (tr ? s.qpre_t[t] : s.qpre_o[t]) + ord;
 	uint32_t slot, vbase, mk, mj, lo[3], hi[3];
