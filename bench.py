@@ -24,6 +24,7 @@ More legs go into the same JSON line:
                     [--no-relight] [--no-dropin]
 """
 import argparse
+import gc
 import json
 import os
 import sys
@@ -338,6 +339,11 @@ def serve_leg(ctx, hcm, workloads, tr, relight=False):
     kinds = {}
     lag = 0.0
     log(f"serving: {len(events)} requests over {events[-1][0]:.2f} s")
+    # the replay follows the trace's request times: a garbage-collection pause of this loop would hold
+    # back thousands of requests and release them as one burst (the server's latency is measured from
+    # each request's submission)
+    gc_was = gc.isenabled()
+    gc.disable()
     t0 = time.perf_counter()
     for tag, (t, kind, payload) in enumerate(events):
         now = time.perf_counter() - t0
@@ -355,6 +361,8 @@ def serve_leg(ctx, hcm, workloads, tr, relight=False):
         kinds[tag] = kind
     srv.drain()
     wall = time.perf_counter() - t0
+    if gc_was:
+        gc.enable()
     done = srv.poll()
     st = srv.stats()
     srv.close()
