@@ -20,7 +20,7 @@ import numpy as np
 
 THREADS = (128, 256, 512)
 WG_PER_CU = (0, 1, 2, 3, 4)  # 0: as many as the LDS footprint allows
-ORDERS = ("linear", "morton", "linear_cost", "morton_cost")
+ORDERS = ("linear", "morton", "linear_cost", "morton_cost", "morton_tailcost")
 QUEUES = 8  # the kernel's job-list parts, one per XCD (hcm_internal.h kQueues)
 
 
@@ -29,7 +29,9 @@ def job_order(name, positions, cost=None):
     3-D Z-order curve of their chunk positions, so each eighth of the list (one XCD's part) is a compact
     block of space and consecutive jobs are neighbours. "<order>_cost" keeps each eighth of <order> and
     sorts the jobs inside it by descending cost (per job, in the caller's order; ties keep <order>), so
-    that each part ends with its cheapest jobs."""
+    that each part ends with its cheapest jobs. "<order>_tailcost" sorts only the second half of each
+    part that way: the first half keeps <order>'s locality, and the part still ends with the cheapest of
+    its last half."""
     p = np.asarray(positions, np.int64)
     base, _, by_cost = name.partition("_")
     if base == "linear":
@@ -43,9 +45,12 @@ def job_order(name, positions, cost=None):
         perm = np.argsort(code, kind="stable")
     if by_cost:
         n = perm.shape[0]
-        c = np.asarray(cost, np.float64)[perm]
-        parts = [perm[q * n // QUEUES:(q + 1) * n // QUEUES][np.argsort(-c[q * n // QUEUES:(q + 1) * n // QUEUES],
-                                                                      kind="stable")] for q in range(QUEUES)]
+        c = np.asarray(cost, np.float64)
+        parts = []
+        for q in range(QUEUES):
+            part = perm[q * n // QUEUES:(q + 1) * n // QUEUES]
+            k = part.shape[0] // 2 if by_cost == "tailcost" else 0  # jobs kept in the base order
+            parts += [part[:k], part[k:][np.argsort(-c[part[k:]], kind="stable")]]
         perm = np.concatenate(parts)
     return perm
 
