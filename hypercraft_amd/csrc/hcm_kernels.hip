@@ -934,14 +934,23 @@ __device__ __forceinline__ void quad_slot(uint32_t pass, uint32_t i, uint32_t n0
 
 // A written quad's box into its mesh's AABB (per lane for the first mesh of a pass, LDS atomics for the
 // others) and, when it starts a continuation mesh, a split event at its emission position epos.
-// A quad's box into the lane's box of the first mesh of its pass (static register indices only).
-__device__ __forceinline__ void box_add(uint32_t box[2][6], uint32_t pass, const uint32_t lo[3], const uint32_t hi[3]) {
+// A quad's box into the lane's box of the first mesh of its pass (static register indices only): the
+// other pass's box takes the empty box (lower corner all ones, upper zero), which changes nothing, and the
+// transparent pass's box is left alone in chunks without transparent quads (any_tr, the same in every lane).
+__device__ __forceinline__ void box_add(uint32_t box[2][6], uint32_t pass, const uint32_t lo[3], const uint32_t hi[3],
+                                        bool any_tr = true) {
+	const uint32_t m = 0u - pass; // all ones for the transparent pass
 #pragma unroll
 	for (int a = 0; a < 3; ++a) {
-		box[0][a] = pass ? box[0][a] : min(box[0][a], lo[a]);
-		box[0][3 + a] = pass ? box[0][3 + a] : max(box[0][3 + a], hi[a]);
-		box[1][a] = pass ? min(box[1][a], lo[a]) : box[1][a];
-		box[1][3 + a] = pass ? max(box[1][3 + a], hi[a]) : box[1][3 + a];
+		box[0][a] = min(box[0][a], lo[a] | m);
+		box[0][3 + a] = max(box[0][3 + a], hi[a] & ~m);
+	}
+	if (any_tr) {
+#pragma unroll
+		for (int a = 0; a < 3; ++a) {
+			box[1][a] = min(box[1][a], lo[a] | ~m);
+			box[1][3 + a] = max(box[1][3 + a], hi[a] & m);
+		}
 	}
 }
 
@@ -972,7 +981,7 @@ __device__ __forceinline__ void quad_done(Smem<NT> &s, uint32_t pass, uint32_t m
 // has more than kSplitQuads quads; without, every quad is in the first mesh of its pass.
 template <bool kSplit, int NT>
 __device__ __forceinline__ void emit_face(const MeshParams &p, Smem<NT> &s, uint32_t desc, uint2 word, uint32_t n0,
-                                          uint32_t pool_base, uint32_t box[2][6]) {
+                                          uint32_t pool_base, uint32_t box[2][6], bool any_tr) {
 	const int t = desc & 127, u = (desc >> 7) & 15, v = (desc >> 11) & 15;
 	const uint32_t so = s.eh[t][u], st = s.tp[t][u], starts = so | st;
 	if (!((starts >> v) & 1u))
@@ -991,21 +1000,22 @@ __device__ __forceinline__ void emit_face(const MeshParams &p, Smem<NT> &s, uint
 		for (int r = u + 2; r < 16 && ((uint32_t(s.ne[t][r] & ~s.ev[t][r]) >> v) & 1u); ++r)
 			++h;
 	}
-	// the pass's starts in rows 0 .. u - 1: eight words of two rows each (all read, no branch); word k
-	// counts whole when 2 k + 2 <= u, its low row only when 2 k + 1 == u
+	// the pass's starts in rows 0 .. u - 1: words of two rows each, word k whole when k < u / 2 (k = 7
+	// never: u <= 15), and the low row of word u / 2 when u is odd
 	const uint32_t *m32 = reinterpret_cast<const uint32_t *>(tr ? s.tp[t] : s.eh[t]);
+	const uint4 wa = reinterpret_cast<const uint4 *>(m32)[0], wb = reinterpret_cast<const uint4 *>(m32)[1];
+	const uint32_t wk[7] = {wa.x, wa.y, wa.z, wa.w, wb.x, wb.y, wb.z};
 	uint32_t ord = __popc((tr ? st : so) & below);
+	const int half = u >> 1;
 #pragma unroll
-	for (int k = 0; k < 8; ++k) {
-		const int d = u - 2 * k; // rows of word k below row u: 2 when d >= 2, 1 when d == 1
-		const uint32_t wk = m32[k];
-		ord += __popc(d >= 2 ? wk : (d == 1 ? wk & 0xffffu : 0u));
-	}
+	for (int k = 0; k < 7; ++k)
+		ord += __popc(k < half ? wk[k] : 0u);
+	ord += __popc(m32[half] & (0u - uint32_t(u & 1)) & 0xffffu);
 	const uint32_t i = (tr ? s.qpre_t[t] : s.qpre_o[t]) + ord;
 	uint32_t slot, vbase, mk, mj, lo[3], hi[3];
 	if (!kSplit) {
 		emit_quad(p, s, t, u, v, w, h, word, pool_base + (tr ? n0 : 0u) + i, 4u * i, lo, hi);
-		box_add(box, tr, lo, hi);
+		box_add(box, tr, lo, hi, any_tr);
 		return;
 	}
 	quad_slot(tr, i, n0, pool_base, slot, vbase, mk, mj);
@@ -1030,7 +1040,7 @@ __device__ __forceinline__ void emit_face(const MeshParams &p, Smem<NT> &s, uint
 // groups with their words computed again.
 template <bool kSplit, int NT>
 __device__ __forceinline__ void emit_regular(const MeshParams &p, Smem<NT> &s, uint32_t *keep, int t0_last, uint32_t n0,
-                                             uint32_t pool_base, uint32_t box[2][6]) {
+                                             uint32_t pool_base, uint32_t box[2][6], bool any_tr) {
 	const int tid = threadIdx.x;
 	constexpr int kMine = (kListCap + NT - 1) / NT, kPairs = (kMine + 1) / 2;
 	{
@@ -1039,9 +1049,9 @@ __device__ __forceinline__ void emit_regular(const MeshParams &p, Smem<NT> &s, u
 		for (int k = 0; k < kPairs; ++k) {
 			const uint32_t i0 = tid + 2 * k * NT, i1 = i0 + NT;
 			if (i0 < n)
-				emit_face<kSplit>(p, s, keep[0] & 0xffffu, s.key[i0], n0, pool_base, box);
+				emit_face<kSplit>(p, s, keep[0] & 0xffffu, s.key[i0], n0, pool_base, box, any_tr);
 			if (i1 < n)
-				emit_face<kSplit>(p, s, keep[0] >> 16, s.key[i1], n0, pool_base, box);
+				emit_face<kSplit>(p, s, keep[0] >> 16, s.key[i1], n0, pool_base, box, any_tr);
 #pragma unroll
 			for (int j = 0; j + 1 < kPairs; ++j)
 				keep[j] = keep[j + 1];
@@ -1070,9 +1080,9 @@ __device__ __forceinline__ void emit_regular(const MeshParams &p, Smem<NT> &s, u
 		for (int k = 0; k < kPairs; ++k) {
 			const uint32_t i0 = tid + 2 * k * NT, i1 = i0 + NT;
 			if (i0 < n)
-				emit_face<kSplit>(p, s, keep[0] & 0xffffu, s.key[i0], n0, pool_base, box);
+				emit_face<kSplit>(p, s, keep[0] & 0xffffu, s.key[i0], n0, pool_base, box, any_tr);
 			if (i1 < n)
-				emit_face<kSplit>(p, s, keep[0] >> 16, s.key[i1], n0, pool_base, box);
+				emit_face<kSplit>(p, s, keep[0] >> 16, s.key[i1], n0, pool_base, box, any_tr);
 #pragma unroll
 			for (int j = 0; j + 1 < kPairs; ++j)
 				keep[j] = keep[j + 1];
@@ -1447,10 +1457,11 @@ template <int NT> __device__ __forceinline__ void mesh_chunk(const MeshParams &p
 	for (int a = 0; a < 6; ++a)
 		box[0][a] = box[1][a] = a < 3 ? ~0u : 0u;
 	// regular quads, one lane per visible face (a quad per face that starts one)
+	const bool any_tr = __builtin_amdgcn_readfirstlane(int(n1)) != 0; // the same in every lane
 	if (n0 > kSplitQuads || n1 > kSplitQuads)
-		emit_regular<true>(p, s, keep, t0_last, n0, pool_base, box);
+		emit_regular<true>(p, s, keep, t0_last, n0, pool_base, box, any_tr);
 	else
-		emit_regular<false>(p, s, keep, t0_last, n0, pool_base, box);
+		emit_regular<false>(p, s, keep, t0_last, n0, pool_base, box, any_tr);
 	// custom quads, after the regular ones (their positions in the output are fixed by the prefix sums): in
 	// windows of the face-word stage, the threads list their cells' quads (cells in YZX order, faces in
 	// mesh order, each thread its range of cells), then one lane per listed quad writes it
