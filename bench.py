@@ -276,12 +276,14 @@ def draw_leg(hcm, workloads, batch, n_meshes, pos):
             "kept_per_camera_occlusion": round(kept_occ / len(cams), 1)}
 
 
-def measured_traffic():
-    """HBM bytes per persistent launch of the cave batch, from the committed rocprofv3 counter summary."""
+def measured_traffic(order):
+    """HBM bytes per persistent launch of the cave batch with its job list in `order`, from the committed
+    rocprofv3 counter summary (None for an order it does not cover)."""
     path = os.path.join(ROOT, "profiles", "mesh_kernel_cave_region.md")
+    key = f"traffic_bytes_per_launch[{order}]:"
     if os.path.exists(path):
         for line in open(path):
-            if line.startswith("traffic_bytes_per_launch:"):
+            if line.startswith(key):
                 return int(line.split(":")[1])
     return None
 
@@ -514,8 +516,9 @@ def main():
         "plain_per_chunk_chunks_per_s": round(plain, 1) if plain else None,
         "batched_vs_plain": round(value / plain, 2) if plain else None,
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 2), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK, 4), "traffic": measured_traffic(),
-                     "traffic_source": "profiles/mesh_kernel_cave_region.md (FETCH_SIZE + WRITE_SIZE, copy-scaled)",
+                     "frac": round(achieved / HBM_PEAK, 4), "traffic": measured_traffic(tuning["order"]),
+                     "traffic_source": f"profiles/mesh_kernel_cave_region.md, job order {tuning['order']} "
+                                       "(FETCH_SIZE + WRITE_SIZE, copy-scaled)",
                      "kernel_ms": round(k_ms, 4), "algorithmic_bytes": int(alg_bytes)},
     }
     res["draw_lists"] = draw_leg(hcm, workloads, batch, int(meshes), positions[perm])

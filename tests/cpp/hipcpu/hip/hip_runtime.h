@@ -6,6 +6,11 @@
 // there -- and __syncthreads completes when every unfinished lane of the workgroup waits at a barrier.
 // LDS is a static object shared by the workgroup's lanes; device memory is host memory; streams run
 // synchronously and a captured graph replays the launches recorded into it.
+// Limit: lanes do not reconverge. A wavefront operation inside a branch that only some lanes take lets the
+// other lanes run on to the next operation after the branch, which can then complete over part of the
+// wavefront (a fold that reads the missing lanes as stale values), where a GPU would run it only after
+// the branch. The kernels keep wavefront operations out of such branches, or make them reached by every
+// lane of the wavefront.
 #pragma once
 
 #include <ucontext.h>
