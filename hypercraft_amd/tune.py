@@ -34,6 +34,10 @@ def job_order(name, positions, cost=None):
     its last half."""
     p = np.asarray(positions, np.int64)
     base, _, by_cost = name.partition("_")
+    if base not in ("linear", "morton") or by_cost not in ("", "cost", "tailcost"):
+        raise ValueError(f"unknown job order {name!r} (orders: {', '.join(ORDERS)})")
+    if by_cost and cost is None:
+        raise ValueError(f"job order {name!r} needs the jobs' costs")
     if base == "linear":
         perm = np.arange(p.shape[0])
     else:

@@ -39,3 +39,18 @@ def test_morton_parts_are_compact_blocks():
         p = pos[perm[q * n // 8:(q + 1) * n // 8]]
         assert (p.max(0) - p.min(0) == 7).all()
     assert tune.job_order("linear", pos).tolist() == list(range(n))
+
+
+def test_unknown_orders_are_refused():
+    pos = workloads.region_job_positions()
+    for bad in ("hilbert", "morton_foo", "linear_tail"):
+        try:
+            tune.job_order(bad, pos, np.zeros(pos.shape[0]))
+        except ValueError:
+            continue
+        raise AssertionError(bad)
+    try:
+        tune.job_order("morton_cost", pos)  # a cost order without costs
+    except ValueError:
+        return
+    raise AssertionError("morton_cost without costs")
