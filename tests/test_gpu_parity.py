@@ -124,6 +124,22 @@ def test_batched_persistent(ctx, golden, name):
     assert_digests(name, digests_of(out), golden[name + "_digest"])
 
 
+def test_job_orders_give_the_same_meshes(ctx, golden):
+    """Every job order the autotuner sweeps (the cost orders with the reference's quad counts as costs)
+    meshes the cave region to the reference's meshes: job k of the reordered batch is region job perm[k]."""
+    from hypercraft_amd import tune
+
+    blocks, light, nbr = gset("cave_region")
+    pos = workloads.region_job_positions()
+    cost = golden["cave_region_quads"].astype(np.float64)
+    for order in tune.ORDERS:
+        perm = tune.job_order(order, pos, cost)
+        say(f"job order {order}: cave_region")
+        out = run_batch(ctx, blocks, light, nbr[perm])
+        assert (out[0]["status"] == 0).all(), order
+        assert_digests(order, digests_of(out), golden["cave_region_digest"][perm])
+
+
 def test_repeated_runs_back_to_back(ctx, golden):
     """hcm_batch_run_repeat: three passes enqueued at once on a fresh batch (its first pass launched on its
     own, then two graph replays) and then five more leave the reference's meshes, and the event time of
