@@ -66,6 +66,33 @@ SMALL = ["lit_glowstone", "lit_terrain_chunk", "lit_palette_all_values", "lit_pa
          "lit_palette_custom", "lit_palette_flow"]
 
 
+@pytest.mark.parametrize("order", ["morton_cost", "morton_tailcost"])
+def test_relit_region_in_tuned_job_orders(ctx, golden, order):
+    """The relit batch in the job orders the tuner applies (the light kernel shares the mesher's job list):
+    job k of the reordered batch rebuilds and meshes region job perm[k] as the reference does (any costs
+    give a valid order; these are seeded)."""
+    from hypercraft_amd import tune
+
+    blocks, heights, nbr = lset("lit_cave_region")
+    cost = np.random.default_rng(7).integers(0, 2000, nbr.shape[0]).astype(np.float64)
+    perm = tune.job_order(order, workloads.region_job_positions(), cost)
+    say(f"light rebuild in order {order}: lit_cave_region")
+    store, batch = relit_batch(ctx, blocks, heights, nbr[perm])
+    try:
+        batch.run()
+        batch.sync()
+        hd = golden["lit_cave_region_halo_digest"][perm]
+        bad = [k for k in range(0, nbr.shape[0], 7)
+               if hashlib.sha256(batch.fetch_light(k).tobytes()).digest() != hd[k].tobytes()]
+        assert not bad, f"{order}: rebuilt light differs in {len(bad)} sampled jobs, first {bad[:10]}"
+        out = batch.download()
+        assert (out[0]["status"] == 0).all()
+        assert_digests(order, digests_of(out), golden["lit_cave_region_mesh_digest"][perm])
+    finally:
+        batch.close()
+        store.close()
+
+
 @pytest.mark.parametrize("name", SMALL + ["lit_cave_region"])
 def test_rebuilt_light_and_meshes_match_reference(ctx, golden, name):
     blocks, heights, nbr = lset(name)
