@@ -239,11 +239,11 @@ __device__ __forceinline__ uint2 light4(const uint32_t *vox, int face, int self)
 	const uint32_t mm = vox[f - sp - sq], mp = vox[f - sp + sq], pmq = vox[f + sp - sq], ppq = vox[f + sp + sq];
 	const bool wa = face == 0 || face == 3 || face == 4;
 	const uint32_t pf = F & kVoxSunTorch, d3 = ((F >> 11) & 1u) << 3;
-	// vertex 1 is corner (+,-) with winding a, else (-,+); vertex 3 the other
-	const uint32_t a1 = wa ? qm : qp, c1 = wa ? pmq : mp, b1 = wa ? pp : pm;
-	const uint32_t a3 = wa ? qp : qm, c3 = wa ? mp : pmq, b3 = wa ? pm : pp;
-	const uint32_t v0 = light_vertex(qm, mm, pm, pf, d3), v1 = light_vertex(a1, c1, b1, pf, d3);
-	const uint32_t v2 = light_vertex(qp, ppq, pp, pf, d3), v3 = light_vertex(a3, c3, b3, pf, d3);
+	// the corners (+,-) and (-,+): vertex 1 is the first with winding a, else the second; vertex 3 the
+	// other (each side cell then enters two vertices through the same expressions)
+	const uint32_t vpm = light_vertex(qm, pmq, pp, pf, d3), vmp = light_vertex(qp, mp, pm, pf, d3);
+	const uint32_t v0 = light_vertex(qm, mm, pm, pf, d3), v1 = wa ? vpm : vmp;
+	const uint32_t v2 = light_vertex(qp, ppq, pp, pf, d3), v3 = wa ? vmp : vpm;
 	const uint32_t lo = (v0 & 63u) | (v1 & 63u) << 6 | (v2 & 63u) << 12 | (v3 & 63u) << 18;
 	const uint32_t hi = ((v0 >> 8) & 63u) | ((v1 >> 8) & 63u) << 6 | ((v2 >> 8) & 63u) << 12 | ((v3 >> 8) & 63u) << 18;
 	const uint32_t ao4 = (v0 >> 16) | (v1 >> 16) << 2 | (v2 >> 16) << 4 | (v3 >> 16) << 6;
